@@ -1,0 +1,154 @@
+"""ctypes binding of libcbc.so (C ABI declared in include/cbc.h).
+
+The library is loaded from ``coupled-bubble-control_amd/lib/libcbc.so``.
+``import torch`` happens first so that the process has a single HIP runtime
+(torch's); the library's DT_NEEDED ``libamdhip64.so.7`` then resolves to it.
+There is no fallback: if the library is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (load torch's HIP runtime before libcbc)
+
+_PKG_ROOT = Path(__file__).resolve().parents[3]
+LIB_PATH = Path(os.environ.get("CBC_LIB", _PKG_ROOT / "lib" / "libcbc.so"))
+
+CBC_ABI_VERSION = 1
+
+CBC_OK = 0
+CBC_ERR_INVALID = -1
+CBC_ERR_HIP = -2
+CBC_ERR_UNSUPPORTED = -3
+CBC_ERR_NOMEM = -4
+CBC_ERR_BUSY = -5
+
+CBC_ACTIVE = 0
+CBC_SUCCESS = 1
+CBC_EVENT_TERMINAL = 2
+CBC_MAX_STEPS_REACHED = -1
+CBC_STEP_SIZE_LIMIT = -2
+CBC_CONVERGENCE_FAILURE = -3
+CBC_SINGULARITY_OR_NAN = -4
+
+CBC_FIELD_CONST = 0
+CBC_FIELD_SW_N = 1
+CBC_FIELD_SW_A = 2
+
+CBC_MAP_AUTO = 0
+CBC_MAP_LANE = 1
+CBC_MAP_GROUP = 2
+
+CBC_LIN_DIRECT = 0
+
+FIELD_CODES = {"CONST": CBC_FIELD_CONST, "SW_N": CBC_FIELD_SW_N, "SW_A": CBC_FIELD_SW_A}
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class ModelSpecC(ctypes.Structure):
+    _fields_ = [("sd", _i32), ("nup", _i32), ("nsp", _i32), ("ncm", _i32), ("nk", _i32),
+                ("ne", _i32), ("ncf", _i32), ("nct", _i32), ("field", _i32)]
+
+
+class ExecSpecC(ctypes.Structure):
+    _fields_ = [("ns", _i32), ("spb", _i32), ("ups", _i32), ("ndo", _i32), ("tile", _i32),
+                ("mapping", _i32), ("block_size", _i32)]
+
+
+class SolverSpecC(ctypes.Structure):
+    _fields_ = [("atol", _f64), ("rtol", _f64), ("min_step", _f64), ("max_step", _f64), ("etol", _f64),
+                ("lin_atol", _f64), ("lin_rtol", _f64), ("lin_relaxation", _f64),
+                ("lin_max_iter", _i32), ("linsolve", _i32), ("col_threshold", _f64)]
+
+
+class BuffersC(ctypes.Structure):
+    _fields_ = [(name, _vp) for name in (
+        "actual_time", "time_end", "time_begin", "time_step", "actual_state",
+        "unit_params", "system_params", "control_params", "coupling_matrices",
+        "dense_index", "dense_time", "dense_state", "actual_event", "status_flags",
+        "total_steps", "rejected_steps")]
+
+
+class PeriodSpecC(ctypes.Structure):
+    _fields_ = [("period", _f64), ("first_window", _i32), ("n_windows", _i32),
+                ("max_r", _vp), ("mean_x", _vp)]
+
+
+class PlanInfoC(ctypes.Structure):
+    _fields_ = [("mapping", _i32), ("block_size", _i32), ("grid_size", _i32), ("systems_per_block", _i32),
+                ("lds_bytes", _i32), ("num_regs", _i32), ("local_bytes", _i32), ("max_threads", _i32)]
+
+
+# name -> (restype, argtypes); every entry point of include/cbc.h
+SIGNATURES = {
+    "cbc_abi_version": (_i32, []),
+    "cbc_last_error": (ctypes.c_char_p, []),
+    "cbc_plan_create": (_i32, [ctypes.POINTER(ModelSpecC), ctypes.POINTER(ExecSpecC),
+                               ctypes.POINTER(SolverSpecC), ctypes.POINTER(_vp)]),
+    "cbc_plan_query": (_i32, [_vp, ctypes.POINTER(PlanInfoC)]),
+    "cbc_plan_destroy": (None, [_vp]),
+    "cbc_launch": (_i32, [_vp, ctypes.POINTER(BuffersC), _i32, _vp]),
+    "cbc_solve": (_i32, [_vp, ctypes.POINTER(BuffersC), _i32, _i64, _vp,
+                         ctypes.POINTER(_f64), _i32, ctypes.POINTER(_i32)]),
+    "cbc_solve_periodic": (_i32, [_vp, ctypes.POINTER(BuffersC), ctypes.POINTER(PeriodSpecC), _i64, _vp]),
+}
+
+
+class CBCError(RuntimeError):
+    """A libcbc call returned a negative status; carries the code and message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libcbc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library (loaded and checked on first use)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"libcbc.so not found at {LIB_PATH}; build it with "
+            "`python coupled-bubble-control_amd/build_lib.py` (or __graft_entry__.build())")
+    handle = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    v = handle.cbc_abi_version()
+    if v != CBC_ABI_VERSION:
+        raise RuntimeError(f"libcbc ABI version {v} != expected {CBC_ABI_VERSION}")
+    _lib = handle
+    return _lib
+
+
+def check(rc: int) -> int:
+    """Raise CBCError for a negative return code."""
+    if rc < 0:
+        msg = lib().cbc_last_error().decode(errors="replace")
+        raise CBCError(rc, msg)
+    return rc
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int | None:
+    """hipStream_t of a torch stream (current stream when None)."""
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream or None

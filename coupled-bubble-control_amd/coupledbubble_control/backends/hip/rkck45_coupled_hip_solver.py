@@ -1,0 +1,136 @@
+"""Solver entry of the HIP backend (reference: backends/cupy_cuda/rkck45_coupled_cupy_solver.py:8-107,
+backends/numba/rkck45_coupled_numba_solver.py:8-125).
+
+``solver`` keeps the reference signature and semantics: up to
+ceil(max_steps/kernel_steps) launches of kernel_steps step attempts each,
+stopping as soon as no system is ACTIVE; with ``benchmark=True`` it returns
+the list of per-launch wall seconds, else None.  The stop test reads one
+device counter (systems left ACTIVE) instead of copying every status flag.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _abi
+
+# trig arguments 2*pi*(f_k/f_ref)*tau + phase must stay below 2^20*pi/2 (device sincos range)
+_MAX_TRIG_ARG = (2.0 ** 20) * (np.pi / 2.0)
+
+
+def _buffers_c(d_buffers) -> _abi.BuffersC:
+    st, pa, de, ss = d_buffers.state, d_buffers.params, d_buffers.dense, d_buffers.status
+    for name, t in (("actual_state", st.actual_state), ("unit_params", pa.unit_params),
+                    ("coupling_matrices", pa.coupling_matrices), ("control_params", pa.control_params)):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name} must be a torch tensor on the HIP device")
+        if t.dtype != torch.float64 or not t.is_contiguous():
+            raise TypeError(f"{name} must be contiguous float64")
+    for name, t in (("status_flags", ss.status_flags), ("dense_index", de.dense_index)):
+        if t.dtype != torch.int32:
+            raise TypeError(f"{name} must be int32")
+    p = _abi.ptr
+    b = _abi.BuffersC()
+    b.actual_time = p(st.actual_time)
+    b.time_end = p(st.time_end)
+    b.time_begin = p(st.time_begin)
+    b.time_step = p(st.time_step)
+    b.actual_state = p(st.actual_state)
+    b.unit_params = p(pa.unit_params)
+    b.system_params = p(pa.system_params)
+    b.control_params = p(pa.control_params)
+    b.coupling_matrices = p(pa.coupling_matrices)
+    b.dense_index = p(de.dense_index) if de.dense_index.numel() else None
+    b.dense_time = p(de.dense_time) if de.dense_time.numel() else None
+    b.dense_state = p(de.dense_state) if de.dense_state.numel() else None
+    b.actual_event = p(ss.actual_event)
+    b.status_flags = p(ss.status_flags)
+    b.total_steps = p(ss.total_steps) if ss.total_steps is not None and ss.total_steps.dtype == torch.int32 else None
+    b.rejected_steps = (p(ss.rejected_steps)
+                        if ss.rejected_steps is not None and ss.rejected_steps.dtype == torch.int32 else None)
+    return b
+
+
+def _check_shapes(d_buffers, model_spec, execution_spec):
+    ns, tile, ups, nk = execution_spec.ns, execution_spec.tile, execution_spec.ups, model_spec.nk
+    want = {
+        "actual_state": (d_buffers.state.actual_state, (4, ns, tile)),
+        "unit_params": (d_buffers.params.unit_params, (13, ns, tile)),
+        "system_params": (d_buffers.params.system_params, (5, ns)),
+        "control_params": (d_buffers.params.control_params, (4 * nk, ns)),
+        "coupling_matrices": (d_buffers.params.coupling_matrices, (3, ups, ns, tile)),
+        "actual_time": (d_buffers.state.actual_time, (ns,)),
+        "time_end": (d_buffers.state.time_end, (ns,)),
+        "time_step": (d_buffers.state.time_step, (ns,)),
+        "actual_event": (d_buffers.status.actual_event, (ns,)),
+        "status_flags": (d_buffers.status.status_flags, (ns,)),
+    }
+    if execution_spec.ndo > 0:
+        want["dense_time"] = (d_buffers.dense.dense_time, (execution_spec.ndo, ns))
+        want["dense_state"] = (d_buffers.dense.dense_state, (execution_spec.ndo, 4, ns, tile))
+        want["dense_index"] = (d_buffers.dense.dense_index, (ns,))
+    for name, (t, shape) in want.items():
+        if tuple(t.shape) != shape:
+            raise ValueError(f"{name} shape mismatch: expected {shape}, got {tuple(t.shape)}")
+
+
+def _check_trig_range(d_buffers, model_spec):
+    nk = model_spec.nk
+    cp = d_buffers.params.control_params
+    sp = d_buffers.params.system_params
+    wmax = (cp[nk:2 * nk].abs() * sp[1:2].abs()).max().item() * 2.0 * np.pi
+    tmax = torch.maximum(d_buffers.state.time_end.abs(), d_buffers.state.actual_time.abs()).max().item()
+    phmax = cp[2 * nk:3 * nk].abs().max().item()
+    if wmax * tmax + phmax >= _MAX_TRIG_ARG:
+        raise ValueError("time horizon too long for the device sincos range "
+                         f"(|theta| up to {wmax * tmax + phmax:.3g} >= {_MAX_TRIG_ARG:.3g})")
+
+
+def solver(kernel, d_buffers, model_spec, execution_spec, solver_spec, max_steps: int,
+           kernel_steps: int = 2048, benchmark: bool = False, debug: bool = False,
+           stream: Optional[torch.cuda.Stream] = None) -> Optional[List[float]]:
+    lib = _abi.lib()
+    _check_shapes(d_buffers, model_spec, execution_spec)
+    _check_trig_range(d_buffers, model_spec)
+    bufs = _buffers_c(d_buffers)
+    max_launches = max(1, (int(max_steps) + int(kernel_steps) - 1) // int(kernel_steps))
+    cap = min(max_launches, 1 << 16)
+    times = (ctypes.c_double * cap)() if benchmark else None
+    n = ctypes.c_int32(0)
+    if debug:
+        print("HIP plan", kernel.info())
+    _abi.check(lib.cbc_solve(kernel.handle, ctypes.byref(bufs), int(kernel_steps), int(max_steps),
+                             _abi.stream_handle(stream), times, cap if benchmark else 0, ctypes.byref(n)))
+    if debug:
+        print("launches", n.value)
+    if benchmark:
+        return [times[i] for i in range(min(n.value, cap))]
+    return None
+
+
+def launch(kernel, d_buffers, kernel_steps: int, stream=None) -> None:
+    """One asynchronous launch of kernel_steps attempts (the reference kernel call itself)."""
+    lib = _abi.lib()
+    bufs = _buffers_c(d_buffers)
+    _abi.check(lib.cbc_launch(kernel.handle, ctypes.byref(bufs), int(kernel_steps), _abi.stream_handle(stream)))
+
+
+def solve_periodic(kernel, d_buffers, period: float, first_window: int, max_r: torch.Tensor,
+                   mean_x: torch.Tensor, max_steps: int = 1 << 40, stream=None) -> None:
+    """Integrate to time_end in windows of ``period`` recording window statistics (asynchronous)."""
+    lib = _abi.lib()
+    _check_shapes(d_buffers, kernel.model_spec, kernel.execution_spec)
+    _check_trig_range(d_buffers, kernel.model_spec)
+    es = kernel.execution_spec
+    nw = int(max_r.shape[0])
+    for name, t in (("max_r", max_r), ("mean_x", mean_x)):
+        if tuple(t.shape) != (nw, es.ns, es.tile) or t.dtype != torch.float64 or not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous float64 of shape {(nw, es.ns, es.tile)}")
+    bufs = _buffers_c(d_buffers)
+    ps = _abi.PeriodSpecC(float(period), int(first_window), nw, _abi.ptr(max_r) if nw else None,
+                          _abi.ptr(mean_x) if nw else None)
+    _abi.check(lib.cbc_solve_periodic(kernel.handle, ctypes.byref(bufs), ctypes.byref(ps), int(max_steps),
+                                      _abi.stream_handle(stream)))

@@ -1,0 +1,58 @@
+// cbc_kernels.h -- launch interface between the C ABI (cbc_api.hip) and the
+// kernel translation units.  Host-only types; no HIP device code here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cbc {
+
+// Arguments of one ensemble launch in the reference buffer layout.
+struct LaunchArgs {
+    int32_t ns, tile, ups, ndo;
+    int32_t nk, field;
+    int64_t kernel_steps;      // attempts per system in this launch
+    double atol, rtol, min_step, max_step, etol;
+    double col_factor;         // 1 + collision threshold
+
+    double* t;
+    const double* t_end;
+    const double* t_begin;
+    double* h;
+    double* state;
+    const double* up;          // unit_params (13, NS, TILE)
+    const double* sp;          // system_params (5, NS)
+    const double* cp;          // control_params (4*NK, NS)
+    const double* cm;          // coupling (3, UPS, NS, TILE)
+    int32_t* dense_index;
+    double* dense_time;
+    double* dense_state;
+    double* event;
+    int32_t* flags;
+    int32_t* total_steps;      // may be null
+    int32_t* rejected_steps;   // may be null
+
+    // period windows (period <= 0: off)
+    double period;
+    int32_t first_window;
+    int32_t n_windows;
+    double* max_r;
+    double* mean_x;
+
+    uint32_t* active_count;    // +1 per system left ACTIVE (may be null)
+};
+
+struct KernelAttr {
+    int32_t num_regs;
+    int32_t local_bytes;
+    int32_t max_threads;
+    int32_t lds_bytes;
+};
+
+// Lane mapping: one system per lane.  Returns hipErrorInvalidValue when no
+// instantiation exists for (ups, nk, field).
+hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream);
+hipError_t lane_attr(int ups, int nk, int field, KernelAttr* out);
+bool lane_supported(int ups, int nk, int field);
+
+}  // namespace cbc

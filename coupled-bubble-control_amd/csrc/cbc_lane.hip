@@ -1,0 +1,486 @@
+// cbc_lane.hip -- lane mapping: one CKM1D system per lane of a wave64.
+//
+// The whole system (4N state values, the six Cash--Karp stages, the unit and
+// coupling constants and the 2N x 2N coupling matrix) lives in VGPRs; there
+// is no LDS, no barrier and no cross-lane traffic, so the lanes of a wave
+// step independent systems with independent step sizes.  The per-system
+// algorithm is that of the reference stepper (rkck45_coupled_numba_stepper.py:179-706,
+// rkck45_coupled_cuda_sover.cu:274-845), which runs one *thread per bubble*
+// and synchronises the block ~20 times per step; here a system never leaves
+// its lane.
+#include "cbc_device.h"
+#include "cbc_kernels.h"
+
+namespace cbc {
+
+template <int N, int NK>
+struct LaneSys {
+    Unit up[N];
+    double rcol[N];           // up[12] = R0 / lambda_ref, collision radius scale
+    double m0[N][N], m1[N][N], m2[N][N];  // m_c[i][j]: effect of bubble j on bubble i
+    double sp0, sp3;          // 3*kappa, liquid sound speed
+    Drive<NK> d;
+};
+
+// In-register LU with partial pivoting of the augmented n x (n+1) system.
+// The pivot search is a sequence of conditional row exchanges so every index
+// stays a compile-time constant (no scratch).  The first row holding the
+// largest |a_ik| ends up as pivot, as in the CPU restatement.  The coupling
+// matrix is I + C with |C| small, so exchanges are rare: the exchange code
+// runs only when some lane of the wave needs one (wave-uniform branch).
+template <int n>
+__device__ __forceinline__ void lu_solve(double (&a)[n][n + 1], double (&x)[n])
+{
+    double inv[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        bool need = false;
+#pragma unroll
+        for (int i = k + 1; i < n; ++i) need |= fabs(a[i][k]) > fabs(a[k][k]);
+        if (__any(need))
+#pragma unroll
+        for (int i = k + 1; i < n; ++i) {
+            const bool sw = fabs(a[i][k]) > fabs(a[k][k]);
+#pragma unroll
+            for (int j = k; j <= n; ++j) {
+                const double u = a[k][j];
+                const double v = a[i][j];
+                a[k][j] = sw ? v : u;
+                a[i][j] = sw ? u : v;
+            }
+        }
+        inv[k] = rcp_fast(a[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < n; ++i) {
+            const double f = a[i][k] * inv[k];
+#pragma unroll
+            for (int j = k + 1; j <= n; ++j) a[i][j] -= f * a[k][j];
+        }
+    }
+#pragma unroll
+    for (int i = n - 1; i >= 0; --i) {
+        double s = a[i][n];
+#pragma unroll
+        for (int j = i + 1; j < n; ++j) s -= a[i][j] * x[j];
+        x[i] = s * inv[i];
+    }
+}
+
+// Full coupled RHS of one system; y, dy ordered [R(N), x(N), U(N), V(N)].
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ void lane_rhs(const LaneSys<N, NK>& P, double t, const double (&y)[4 * N],
+                                         double (&dy)[4 * N])
+{
+    double st[NK], ct[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) sincos_fast(P.d.wt[k] * t + P.d.ph[k], &st[k], &ct[k]);
+
+    BubbleOut o[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double pa, pat, pax, uax;
+        field_at<NK, FIELD>(P.d, st, ct, y[N + i], pa, pat, pax, uax);
+        bubble_rhs(P.up[i], P.sp0, P.sp3, y[i], y[2 * N + i], y[3 * N + i], pa, pat, pax, uax, o[i]);
+        dy[i] = y[2 * N + i];
+        dy[N + i] = y[3 * N + i];
+    }
+
+    if constexpr (N == 1) {
+        dy[2] = o[0].dU;
+        dy[3] = o[0].dV;
+    } else {
+        // pair geometry, one division per unordered pair
+        double rd[N][N], sg[N][N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+#pragma unroll
+            for (int j = i + 1; j < N; ++j) {
+                const double delta = y[N + i] - y[N + j];
+                const double s = sgn(delta);
+                const double r = rcp_fast(fmax(fabs(delta), 1e-30));
+                rd[i][j] = r; rd[j][i] = r;
+                sg[i][j] = s; sg[j][i] = -s;
+            }
+        }
+        // explicit coupling (ckm1d_def.cuh:103-166) and the implicit system
+        // (ckm1d_def.cuh:213-248) assembled as [I + C | b]
+        double a[2 * N][2 * N + 1];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double rg0 = 0.0, rg1 = 0.0, tng = 0.0, tg2 = 0.0, tg3 = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (j == i) {
+                    a[i][j] = 1.0;
+                    a[i][N + j] = 0.0;
+                    a[N + i][j] = 0.0;
+                    a[N + i][N + j] = 1.0;
+                    continue;
+                }
+                const double r = rd[i][j];
+                const double r2 = r * r;
+                const double sr2 = r2 * sg[i][j];
+                const double r3 = r2 * r;
+                const double m0 = P.m0[i][j], m1 = P.m1[i][j], m2 = P.m2[i][j];
+                rg0 += m0 * (-2.0 * r * o[j].H0 - 2.5 * sr2 * o[j].H2 - 1.0 * r3 * o[j].H4);
+                rg1 += m0 * (-0.5 * sr2 * o[j].H1 - 0.5 * r3 * o[j].H3);
+                tng += m1 * (2.0 * sr2 * o[j].H0 + 5.0 * r3 * o[j].H2);
+                tg2 += m1 * (sr2 * o[j].H1 + r3 * o[j].H3);
+                tg3 += m2 * (sr2 * o[j].H1 + r3 * o[j].H3);
+                a[i][j] = o[i].G0 * m0 * r * o[j].H5;
+                a[i][N + j] = 0.5 * o[i].G0 * m0 * sr2 * o[j].H6;
+                a[N + i][j] = -m1 * sr2 * o[j].H5;
+                a[N + i][N + j] = -m1 * r3 * o[j].H6;
+            }
+            a[i][2 * N] = o[i].dU + (rg0 * o[i].G0 + rg1 * o[i].G1);
+            a[N + i][2 * N] = o[i].dV + (tng + tg2 * o[i].G2 + tg3 * o[i].G3);
+        }
+        double v[2 * N];
+        lu_solve<2 * N>(a, v);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            dy[2 * N + i] = v[i];
+            dy[3 * N + i] = v[N + i];
+        }
+    }
+}
+
+// Collision gap min_i |x_i - x_{i+1}| - colf (r_i + r_{i+1}), r = R/R0 * R0/lambda;
+// ckm1d_def.py:512-552.  A single bubble has no gap (1e30).
+template <int N, int NK>
+__device__ __forceinline__ double lane_event(const LaneSys<N, NK>& P, const double (&y)[4 * N], double colf)
+{
+    double g = 1e30;
+#pragma unroll
+    for (int i = 0; i + 1 < N; ++i) {
+        const double ri = y[i] * P.rcol[i];
+        const double rj = y[i + 1] * P.rcol[i + 1];
+        const double gap = fabs(y[N + i] - y[N + i + 1]) - (ri + rj) * colf;
+        if (gap < g) g = gap;
+    }
+    return g;
+}
+
+template <int N, int NK, int FIELD>
+__global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.ns) return;
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    const size_t NST = NS * TILE;
+    const size_t base = (size_t)s * TILE;
+
+    // load and early termination (rkck45_coupled_numba_stepper.py:179-213)
+    double t = a.t[s];
+    const double t_end = a.t_end[s];
+    double h = a.h[s];
+    double g_old = a.event[s];
+    {
+        bool term = t >= t_end;
+        int flag = term ? kSuccess : kActive;
+        if (fabs(g_old) < a.etol) {
+            term = true;
+            flag = kEventTerminal;
+        }
+        if (term) {
+            a.flags[s] = flag;
+            return;
+        }
+    }
+
+    LaneSys<N, NK> P;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) P.up[i].u[j] = a.up[j * NST + base + i];
+        P.rcol[i] = a.up[12 * NST + base + i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            // coupling[c][j][s][i] is the effect of unit j on unit i (ckm1d_def.cuh:56-57)
+            const size_t off = ((size_t)j * NS + s) * TILE + i;
+            const size_t cstride = (size_t)a.ups * NST;
+            P.m0[i][j] = a.cm[off];
+            P.m1[i][j] = a.cm[cstride + off];
+            P.m2[i][j] = a.cm[2 * cstride + off];
+        }
+    }
+    const double sp0 = a.sp[s], sp1 = a.sp[NS + s], sp2 = a.sp[2 * NS + s];
+    const double sp3 = a.sp[3 * NS + s], sp4 = a.sp[4 * NS + s];
+    P.sp0 = sp0;
+    P.sp3 = sp3;
+    {
+        const double TWO_PI = 6.28318530717958647692;
+        const double w0 = TWO_PI * sp1;
+        const double l0 = TWO_PI * sp2;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const double amps = a.cp[(size_t)k * NS + s];
+            const double omega = a.cp[(size_t)(NK + k) * NS + s];
+            const double phase = a.cp[(size_t)(2 * NK + k) * NS + s];
+            const double wave = a.cp[(size_t)(3 * NK + k) * NS + s];
+            P.d.wt[k] = w0 * omega;
+            P.d.wx[k] = l0 * wave;
+            P.d.ph[k] = phase;
+            P.d.A[k] = amps;
+            P.d.Aw[k] = amps * omega;
+            P.d.Ak[k] = amps * wave;
+            P.d.Au[k] = amps * sp4;
+        }
+    }
+
+    double y[4 * N];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) y[j * N + i] = a.state[j * NST + base + i];
+
+    g_old = lane_event<N, NK>(P, y, a.col_factor);
+
+    // dense output (rkck45_coupled_numba_stepper.py:72-112, 333-345)
+    const bool dense = a.ndo > 0;
+    int didx = 0;
+    double dnext = t, ddt = 0.0;
+    if (dense) {
+        const double tb = a.t_begin[s];
+        didx = a.dense_index[s];
+        ddt = (a.ndo == 1) ? (t_end - tb) : (t_end - tb) / (a.ndo - 1);
+        if (didx < a.ndo && dnext <= t) {
+            a.dense_time[(size_t)didx * NS + s] = t;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    a.dense_state[(((size_t)didx * 4 + j) * NS + s) * TILE + i] = y[j * N + i];
+            ++didx;
+            dnext = fmin(t + ddt, t_end);
+        }
+    }
+
+    // period windows: a window end is an intermediate end time of the clamp
+    const bool win = a.period > 0.0;
+    int wcur = 0;
+    double wend = t_end;
+    double wmax[N], wsum[N];
+    if (win) {
+        wcur = (int)floor(t / a.period + 1e-12);
+        wend = fmin((wcur + 1) * a.period, t_end);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            wmax[i] = y[i];
+            wsum[i] = 0.0;
+        }
+    }
+
+    int flag = kActive;
+    int attempts = 0, rejected = 0;
+    for (int64_t step = 0; step < a.kernel_steps; ++step) {
+        ++attempts;
+        // clamp (rkck45_coupled_numba_stepper.py:366-373)
+        if (h < a.min_step) h = a.min_step;
+        if (h > a.max_step) h = a.max_step;
+        {
+            const double remain = wend - t;
+            if (h > remain) h = remain;
+        }
+
+        // Cash--Karp stages (rkck45_coupled_numba_stepper.py:379-484)
+        double k1[4 * N], k2[4 * N], k3[4 * N], k4[4 * N], k5[4 * N], k6[4 * N], tmp[4 * N];
+        lane_rhs<N, NK, FIELD>(P, t, y, k1);
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
+        lane_rhs<N, NK, FIELD>(P, t + CK::a2 * h, tmp, k2);
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
+        lane_rhs<N, NK, FIELD>(P, t + CK::a3 * h, tmp, k3);
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c)
+            tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
+        lane_rhs<N, NK, FIELD>(P, t + CK::a4 * h, tmp, k4);
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c)
+            tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
+        lane_rhs<N, NK, FIELD>(P, t + CK::a5 * h, tmp, k5);
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c)
+            tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
+                                 CK::b65 * k5[c]);
+        lane_rhs<N, NK, FIELD>(P, t + CK::a6 * h, tmp, k6);
+
+        double ynew[4 * N];
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c)
+            ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
+                                  CK::c5 * k5[c] + CK::c6 * k6[c]);
+        const double g_new = lane_event<N, NK>(P, ynew, a.col_factor);
+
+        // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
+        const bool crossing = g_old * g_new < 0.0;
+        if (crossing) {
+            const double aold = fabs(g_old);
+            const double ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
+            h = h * ratio;
+            ++rejected;
+        } else {
+            // error norm (rkck45_coupled_numba_stepper.py:525-536)
+            double ratio = 1.0e30;
+            bool bad = false;
+#pragma unroll
+            for (int c = 0; c < 4 * N; ++c) {
+                const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
+                                           CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
+                const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
+                const double r = tol * rcp_fast(e);
+                bad |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);
+                if (r < ratio) ratio = r;
+            }
+            if (bad) {
+                flag = kNaN;
+                break;
+            }
+            const bool ok = ratio >= 1.0;
+            const double fac = step_factor(ok, ratio);
+            if (ok) {
+                if (win) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        if (ynew[i] > wmax[i]) wmax[i] = ynew[i];
+                        wsum[i] += 0.5 * h * (y[N + i] + ynew[N + i]);
+                    }
+                }
+                t = t + h;
+#pragma unroll
+                for (int c = 0; c < 4 * N; ++c) y[c] = ynew[c];
+                g_old = g_new;
+            } else {
+                ++rejected;
+            }
+            h = h * fac;
+        }
+
+        if (dense && didx < a.ndo && dnext <= t) {
+            a.dense_time[(size_t)didx * NS + s] = t;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    a.dense_state[(((size_t)didx * 4 + j) * NS + s) * TILE + i] = y[j * N + i];
+            ++didx;
+            dnext = fmin(t + ddt, t_end);
+        }
+
+        if (win && t >= wend) {
+            const int widx = wcur - a.first_window;
+            if (widx >= 0 && widx < a.n_windows) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax[i];
+                    a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum[i] / a.period;
+                }
+            }
+            ++wcur;
+            wend = fmin((wcur + 1) * a.period, t_end);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                wmax[i] = y[i];
+                wsum[i] = 0.0;
+            }
+        }
+
+        // termination (rkck45_coupled_numba_stepper.py:651-663)
+        if (t >= t_end) {
+            flag = kSuccess;
+            break;
+        }
+        if (fabs(g_old) < a.etol) {
+            flag = kEventTerminal;
+            break;
+        }
+    }
+
+    // store (rkck45_coupled_numba_stepper.py:683-704)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.state[j * NST + base + i] = y[j * N + i];
+    a.t[s] = t;
+    a.h[s] = h;
+    a.event[s] = g_old;
+    a.flags[s] = flag;
+    if (dense) a.dense_index[s] = didx;
+    if (a.total_steps) a.total_steps[s] += attempts;
+    if (a.rejected_steps) a.rejected_steps[s] += rejected;
+    if (flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <int N, int NK, int FIELD>
+static const void* lane_fn()
+{
+    return reinterpret_cast<const void*>(&lane_kernel<N, NK, FIELD>);
+}
+
+template <int N, int NK>
+static const void* lane_pick_field(int field)
+{
+    switch (field) {
+    case kFieldConst: return lane_fn<N, NK, kFieldConst>();
+    case kFieldSwN: return lane_fn<N, NK, kFieldSwN>();
+    case kFieldSwA: return lane_fn<N, NK, kFieldSwA>();
+    default: return nullptr;
+    }
+}
+
+template <int N>
+static const void* lane_pick_nk(int nk, int field)
+{
+    switch (nk) {
+    case 1: return lane_pick_field<N, 1>(field);
+    case 2: return lane_pick_field<N, 2>(field);
+    default: return nullptr;
+    }
+}
+
+static const void* lane_pick(int ups, int nk, int field)
+{
+    switch (ups) {
+    case 1: return lane_pick_nk<1>(nk, field);
+    case 2: return lane_pick_nk<2>(nk, field);
+    case 3: return lane_pick_nk<3>(nk, field);
+    case 4: return lane_pick_nk<4>(nk, field);
+    default: return nullptr;
+    }
+}
+
+bool lane_supported(int ups, int nk, int field) { return lane_pick(ups, nk, field) != nullptr; }
+
+hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream)
+{
+    const void* fn = lane_pick(a.ups, a.nk, a.field);
+    if (!fn) return hipErrorInvalidValue;
+    if (block <= 0 || block > 256 || (block % 64) != 0) return hipErrorInvalidValue;
+    const int grid = (a.ns + block - 1) / block;
+    void* args[] = {const_cast<LaunchArgs*>(&a)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+}
+
+hipError_t lane_attr(int ups, int nk, int field, KernelAttr* out)
+{
+    const void* fn = lane_pick(ups, nk, field);
+    if (!fn) return hipErrorInvalidValue;
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    out->num_regs = fa.numRegs;
+    out->local_bytes = (int32_t)fa.localSizeBytes;
+    out->max_threads = fa.maxThreadsPerBlock;
+    out->lds_bytes = (int32_t)fa.sharedSizeBytes;
+    return hipSuccess;
+}
+
+}  // namespace cbc

@@ -1,0 +1,179 @@
+/*
+ * cbc.h -- C ABI of libcbc.so, the MI355X (gfx950) ensemble solver for the
+ * coupled Keller--Miksis 1-D bubble model (CKM1D).
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success or a
+ * negative CBC_ERR_* code, and never lets an exception cross the boundary.
+ * cbc_last_error() gives a readable message for the calling thread's last
+ * failure.  Device buffers are owned by the caller (any HIP allocation, e.g.
+ * torch tensors); a plan owns only its code selection and a small workspace.
+ * Work is stream-ordered on the hipStream_t passed as `stream` (NULL = the
+ * legacy default stream).  One host thread per plan.
+ *
+ * Buffer layouts are the reference's (coupledbubble_control/models/model.py:56-106),
+ * all FP64 unless stated:
+ *   actual_time, time_end, time_begin, time_step, actual_event   (NS,)
+ *   status_flags                                                 (NS,)   int32
+ *   actual_state                                                 (4, NS, TILE)
+ *   unit_params                                                  (13, NS, TILE)
+ *   system_params                                                (5, NS)
+ *   control_params                                               (4*NK, NS)
+ *   coupling_matrices                                            (3, UPS, NS, TILE)
+ *   dense_index (NS,) int32, dense_time (NDO, NS), dense_state (NDO, 4, NS, TILE)
+ * TILE = next power of two >= UPS (models/specs.py:47).
+ */
+#ifndef CBC_H
+#define CBC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBC_ABI_VERSION 1
+
+/* return codes */
+#define CBC_OK 0
+#define CBC_ERR_INVALID (-1)      /* bad argument, shape or spec */
+#define CBC_ERR_HIP (-2)          /* a HIP runtime call failed */
+#define CBC_ERR_UNSUPPORTED (-3)  /* configuration has no kernel */
+#define CBC_ERR_NOMEM (-4)        /* host or device allocation failed */
+#define CBC_ERR_BUSY (-5)         /* serving queue full / admission refused */
+
+/* per-system status flags; rkck45_coupled_numba_stepper.py:27-33 */
+#define CBC_ACTIVE 0
+#define CBC_SUCCESS 1
+#define CBC_EVENT_TERMINAL 2
+#define CBC_MAX_STEPS_REACHED (-1)
+#define CBC_STEP_SIZE_LIMIT (-2)
+#define CBC_CONVERGENCE_FAILURE (-3)
+#define CBC_SINGULARITY_OR_NAN (-4)
+
+/* acoustic field (ModelSpec.ac, models/specs.py:16) */
+#define CBC_FIELD_CONST 0
+#define CBC_FIELD_SW_N 1
+#define CBC_FIELD_SW_A 2
+
+/* system-to-hardware mapping */
+#define CBC_MAP_AUTO 0
+#define CBC_MAP_LANE 1   /* one system per lane, whole state in VGPRs */
+#define CBC_MAP_GROUP 2  /* TILE lanes per system, coupling solve staged in LDS */
+
+/* coupling solve */
+#define CBC_LIN_DIRECT 0      /* LU with partial pivoting */
+
+/* ModelSpec, models/specs.py:5-23 (name and ac are replaced by `field`) */
+typedef struct cbc_model_spec {
+    int32_t sd, nup, nsp, ncm, nk, ne, ncf, nct;
+    int32_t field;
+} cbc_model_spec;
+
+/* ExecutionSpec, models/specs.py:26-52, plus the mapping request */
+typedef struct cbc_exec_spec {
+    int32_t ns, spb, ups, ndo, tile;
+    int32_t mapping;       /* CBC_MAP_* */
+    int32_t block_size;    /* threads per block, 0 = plan default */
+} cbc_exec_spec;
+
+/* SolverSpec, models/specs.py:55-71, plus the collision threshold of
+ * ckm1d_def.py:534 and the solve selector */
+typedef struct cbc_solver_spec {
+    double atol, rtol, min_step, max_step, etol;
+    double lin_atol, lin_rtol, lin_relaxation;
+    int32_t lin_max_iter;
+    int32_t linsolve;       /* CBC_LIN_* */
+    double col_threshold;
+} cbc_solver_spec;
+
+/* The 14 kernel buffers of rkck45_coupled_cuda_sover.cu:206-226 plus the two
+ * step counters of models/buffers.py:57-63 (int32, may be NULL). */
+typedef struct cbc_buffers {
+    double *actual_time;
+    const double *time_end;
+    const double *time_begin;
+    double *time_step;
+    double *actual_state;
+    const double *unit_params;
+    const double *system_params;
+    const double *control_params;
+    const double *coupling_matrices;
+    int32_t *dense_index;
+    double *dense_time;
+    double *dense_state;
+    double *actual_event;
+    int32_t *status_flags;
+    int32_t *total_steps;
+    int32_t *rejected_steps;
+} cbc_buffers;
+
+/* Per-window statistics for parameter studies: windows of `period` tau;
+ * windows with index < first_window are skipped; outputs are
+ * (n_windows, NS, TILE): maximum R/R0 over the accepted states of the window
+ * (including its first state) and the trapezoidal time-mean of x/lambda. */
+typedef struct cbc_period_spec {
+    double period;
+    int32_t first_window;
+    int32_t n_windows;
+    double *max_r;
+    double *mean_x;
+} cbc_period_spec;
+
+typedef struct cbc_plan_info {
+    int32_t mapping;          /* CBC_MAP_LANE or CBC_MAP_GROUP */
+    int32_t block_size;
+    int32_t grid_size;
+    int32_t systems_per_block;
+    int32_t lds_bytes;
+    int32_t num_regs;         /* from hipFuncGetAttributes */
+    int32_t local_bytes;      /* private (spill) bytes per lane */
+    int32_t max_threads;
+} cbc_plan_info;
+
+typedef struct cbc_plan cbc_plan;
+
+/* ABI version of the loaded library (CBC_ABI_VERSION). */
+int32_t cbc_abi_version(void);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *cbc_last_error(void);
+
+/* Selects and sizes the kernel for a model/execution/solver spec; replaces
+ * load_coupled_kernel (backends/cupy_cuda/kernel_loader.py:9-99) and the
+ * Numba make_model/make_kernel pair (models/ckm1d.py:214-232).  Allocates a
+ * 16-byte device workspace (the active-system counter of cbc_solve). */
+int32_t cbc_plan_create(const cbc_model_spec *model, const cbc_exec_spec *exec,
+                        const cbc_solver_spec *solver, cbc_plan **out);
+
+/* Reports the kernel chosen by a plan. */
+int32_t cbc_plan_query(const cbc_plan *plan, cbc_plan_info *out);
+
+void cbc_plan_destroy(cbc_plan *plan);
+
+/* One launch of up to kernel_steps step attempts per system; the device
+ * entry rkck45_coupled_solver (rkck45_coupled_cuda_sover.cu:206-846).
+ * Asynchronous on `stream`. */
+int32_t cbc_launch(cbc_plan *plan, const cbc_buffers *buf, int32_t kernel_steps, void *stream);
+
+/* The host launch loop of rkck45_coupled_cupy_solver.py:29-80: up to
+ * ceil(max_steps/kernel_steps) launches, stopping once no system is ACTIVE.
+ * The stop test reads one device counter instead of every flag.  When
+ * launch_times is not NULL it receives the wall seconds of each launch (at
+ * most max_times entries).  *n_launches receives the launch count.  Returns
+ * after the stream is synchronised. */
+int32_t cbc_solve(cbc_plan *plan, const cbc_buffers *buf, int32_t kernel_steps, int64_t max_steps,
+                  void *stream, double *launch_times, int32_t max_times, int32_t *n_launches);
+
+/* One launch that integrates every system to time_end in windows of
+ * period->period tau, recording window statistics (no counterpart in the
+ * reference; its per-period outputs are what the parameter-study scripts
+ * derive from dense output, scripts/verify_CKM1D_paramstudy.py:268-330).
+ * max_steps bounds the attempts per system.  Asynchronous on `stream`. */
+int32_t cbc_solve_periodic(cbc_plan *plan, const cbc_buffers *buf, const cbc_period_spec *period,
+                           int64_t max_steps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBC_H */

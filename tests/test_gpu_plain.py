@@ -1,0 +1,245 @@
+"""GPU parity of the plain (one launch per batch) HIP solver against the C oracle.
+
+Both run the reference algorithm (rkck45_coupled_numba_stepper.py:116-706 with
+the CKM1D RHS) on identical reference-layout inputs; they differ only by
+rounding (FMA contraction, device elementary functions within 2 ulp), so on
+the short non-chaotic windows used here final states agree far inside the
+1e-6 relative bar and step counts agree within 2%.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6          # final-state bar: relative, per quantity (R, x, U, V) and system, in the
+ATOL_FLOOR = 1e-9    # max norm over the system's bubbles, floored at FLOOR_FRAC of the quantity's
+FLOOR_FRAC = 1e-3    # ensemble maximum (velocities cross zero; a system at rest has U = V = 0)
+
+
+def _scale(want):
+    """Per (quantity, system) scale for a (4, NS, n) state array."""
+    sys_max = np.abs(want).max(axis=2, keepdims=True)
+    ens_max = np.abs(want).max(axis=(1, 2), keepdims=True)
+    return np.maximum(sys_max, FLOOR_FRAC * ens_max) + ATOL_FLOOR
+
+
+def _run_pair(w, kernel_steps=2048, max_steps=10 ** 8, ndo=0, variant=None):
+    import torch  # noqa: F401
+
+    from coupledbubble_control import workloads
+    from oracle import ckm1d_oracle as O
+
+    m = workloads.build_model(w, num_stored_points=ndo, variant=variant)
+    ref = O.host_buffers_dict(m)
+    spec = O.spec_from_model(m)
+    cnt = O.solve(spec, ref, kernel_steps=kernel_steps, max_steps=max_steps)
+    m.solve(max_steps=max_steps, kernel_steps=kernel_steps, sync_to_host=True)
+    return m, ref, cnt
+
+
+def _assert_states(m, ref, n, rtol=RTOL):
+    got = m.runtime.host_buffers.state.actual_state[:, :, :n]
+    want = ref["actual_state"][:, :, :n]
+    err = np.abs(got - want) / _scale(want)
+    worst = np.unravel_index(np.argmax(err), err.shape)
+    assert err.max() <= rtol, f"max rel err {err.max():.3e} at {worst}: got {got[worst]} want {want[worst]}"
+    return err.max()
+
+
+def _assert_counts(m, cnt, frac=0.02):
+    got = m.runtime.host_buffers.status.total_steps.astype(np.int64)
+    want = cnt["total_steps"]
+    rel = np.abs(got - want) / np.maximum(want, 1)
+    assert rel.max() <= frac, f"step-count mismatch {rel.max():.3%} (got {got[np.argmax(rel)]}, want {want[np.argmax(rel)]})"
+    total_rel = abs(int(got.sum()) - int(want.sum())) / max(int(want.sum()), 1)
+    assert total_rel <= frac / 4
+    return rel.max()
+
+
+def test_plain_256xN2_one_rl_step(device):
+    """256 systems of N=2, dual-frequency standing wave, 5 periods, rtol=atol=1e-9."""
+    from coupledbubble_control import workloads
+
+    print("case: plain_cpu 256xN2 tau=5", flush=True)
+    w = workloads.plain_cpu(seed=0)
+    m, ref, cnt = _run_pair(w)
+    hb = m.runtime.host_buffers
+    assert np.array_equal(hb.status.status_flags, ref["status_flags"])
+    assert set(np.unique(hb.status.status_flags)) <= {1, 2}
+    err = _assert_states(m, ref, 2)
+    _assert_counts(m, cnt)
+    done = hb.status.status_flags == 1
+    np.testing.assert_array_equal(hb.state.actual_time[done], ref["actual_time"][done])
+    # collision times are located by the step controller; they agree to the state bar
+    np.testing.assert_allclose(hb.state.actual_time[~done], ref["actual_time"][~done], rtol=RTOL)
+    np.testing.assert_allclose(hb.status.actual_event, ref["actual_event"], rtol=1e-6, atol=1e-9)
+    print(f"max rel err {err:.2e}")
+
+
+@pytest.mark.parametrize("field", ["CONST", "SW_A"])
+def test_field_types(device, field):
+    from coupledbubble_control import workloads
+
+    print(f"case: field {field} 64xN2 tau=2", flush=True)
+    w = workloads.plain_cpu(seed=3, ns=64)
+    w.field = field
+    w.t_end[:] = 2.0
+    m, ref, cnt = _run_pair(w)
+    assert np.array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+    _assert_states(m, ref, 2)
+    _assert_counts(m, cnt)
+
+
+def test_single_harmonic_and_single_bubble(device):
+    """NK=1 drive, and N=1 (no coupling) systems."""
+    from coupledbubble_control import workloads
+
+    print("case: NK=1 64xN2, N=1 64x1", flush=True)
+    w = workloads.plain_cpu(seed=5, ns=64)
+    w.PA, w.FR, w.PS = w.PA[:1], w.FR[:1], w.PS[:1]
+    w.t_end[:] = 2.0
+    m, ref, cnt = _run_pair(w)
+    _assert_states(m, ref, 2)
+    _assert_counts(m, cnt)
+    w1 = workloads.plain_cpu(seed=6, ns=64)
+    w1.n = 1
+    w1.R0, w1.X0 = w1.R0[:, :1], w1.X0[:, :1]
+    w1.t_end[:] = 2.0
+    m, ref, cnt = _run_pair(w1)
+    _assert_states(m, ref, 1)
+    _assert_counts(m, cnt)
+
+
+def test_relaunch_is_resumable(device):
+    """Many short launches give the same result as one long launch (state, t, h and
+    event persist in HBM between launches; rkck45_coupled_cuda_sover.cu:274-287, 820-845)."""
+    from coupledbubble_control import workloads
+
+    print("case: relaunch kernel_steps 37 vs 100000, 128xN2 tau=2", flush=True)
+    w = workloads.plain_cpu(seed=7, ns=128)
+    w.t_end[:] = 2.0
+    m1 = workloads.build_model(w)
+    m1.solve(max_steps=10 ** 8, kernel_steps=37, sync_to_host=True)
+    m2 = workloads.build_model(w)
+    m2.solve(max_steps=10 ** 8, kernel_steps=100000, sync_to_host=True)
+    a, b = m1.runtime.host_buffers, m2.runtime.host_buffers
+    np.testing.assert_array_equal(a.state.actual_state, b.state.actual_state)
+    np.testing.assert_array_equal(a.state.actual_time, b.state.actual_time)
+    np.testing.assert_array_equal(a.state.time_step, b.state.time_step)
+    np.testing.assert_array_equal(a.status.total_steps, b.status.total_steps)
+
+
+def test_identical_systems_identical_results(device):
+    """Same IVP in every system gives bit-identical outputs (the invariant of
+    scripts/verify_CKM1D_pointwise_fixed_ups.py:162-203, there checked to 1e-12)."""
+    from coupledbubble_control import workloads
+
+    print("case: 512 identical systems", flush=True)
+    w = workloads.plain_cpu(seed=8, ns=1)
+    w = w.subset(np.zeros(512, dtype=int))
+    w.t_end[:] = 3.0
+    m = workloads.build_model(w)
+    m.solve(max_steps=10 ** 8, kernel_steps=2048, sync_to_host=True)
+    st = m.runtime.host_buffers.state.actual_state
+    assert np.all(st == st[:, :1, :])
+
+
+def test_max_steps_leaves_active(device):
+    """max_steps below the need leaves flags ACTIVE (0), which the env treats as failure."""
+    from coupledbubble_control import workloads
+
+    print("case: max_steps=64", flush=True)
+    w = workloads.plain_cpu(seed=9, ns=64)
+    m, ref, cnt = _run_pair(w, kernel_steps=16, max_steps=64)
+    assert np.all(m.runtime.host_buffers.status.status_flags == 0)
+    assert np.all(m.runtime.host_buffers.status.total_steps == 64)
+    _assert_states(m, ref, 2)
+
+
+def test_event_already_reached(device):
+    """|event| < ETOL at load terminates immediately with EVENT_TERMINAL
+    (rkck45_coupled_numba_stepper.py:203-211)."""
+    from coupledbubble_control import workloads
+
+    print("case: event preset", flush=True)
+    w = workloads.plain_cpu(seed=10, ns=64)
+    m = workloads.build_model(w)
+    m.runtime.host_buffers.status.actual_event[::2] = 0.0
+    m.solve(max_steps=10 ** 7, kernel_steps=2048, sync_to_host=True)
+    hb = m.runtime.host_buffers
+    assert np.all(hb.status.status_flags[::2] == 2)
+    assert np.all(hb.state.actual_time[::2] == 0.0)
+    assert np.all(hb.status.status_flags[1::2] >= 1)
+
+
+def test_nan_state_is_flagged(device):
+    """A non-finite step reports SINGULARITY_OR_NAN (-4) instead of SUCCESS."""
+    from coupledbubble_control import workloads
+
+    print("case: nan injection", flush=True)
+    w = workloads.plain_cpu(seed=11, ns=64)
+    m = workloads.build_model(w)
+    m.runtime.host_buffers.state.actual_state[0, 5, 0] = np.nan
+    m.solve(max_steps=10 ** 7, kernel_steps=2048, sync_to_host=True)
+    flags = m.runtime.host_buffers.status.status_flags
+    assert flags[5] == -4
+    assert np.all(np.delete(flags, 5) >= 1)
+
+
+def test_dense_output(device):
+    from coupledbubble_control import workloads
+
+    print("case: dense output ndo=16", flush=True)
+    w = workloads.plain_cpu(seed=12, ns=64)
+    w.t_end[:] = 2.0
+    m, ref, cnt = _run_pair(w, ndo=16)
+    hb = m.runtime.host_buffers
+    np.testing.assert_array_equal(hb.dense.dense_index, ref["dense_index"])
+    di = hb.dense.dense_index
+    assert np.all((di >= 0) & (di <= 16))
+    for s in range(w.ns):
+        k = di[s]
+        # sample times are accepted-step times (sums of controller step sizes)
+        np.testing.assert_allclose(hb.dense.dense_time[:k, s], ref["dense_time"][:k, s], rtol=RTOL)
+        # a state sampled at a time offset dt differs by ~ rate * dt: bound by the
+        # local rate from neighbouring samples (plus the state bar)
+        got = hb.dense.dense_state[:k, :, s, :2]
+        want = ref["dense_state"][:k, :, s, :2]
+        tg, tw = hb.dense.dense_time[:k, s], ref["dense_time"][:k, s]
+        rate = np.abs(np.gradient(want, tw, axis=0)) if k > 2 else np.zeros_like(want)
+        scale = np.abs(want).max(axis=(0, 2), keepdims=True) + ATOL_FLOOR
+        bound = RTOL * scale + 4.0 * rate * np.abs(tg - tw)[:, None, None]
+        assert np.all(np.abs(got - want) <= bound)
+
+
+def test_period_windows(device):
+    """Window statistics (max R/R0, mean x per period) on a small parameter grid.
+
+    mean x is a trapezoidal integral and agrees to the 1e-6 bar.  max R/R0 is a
+    maximum over accepted-step samples, so it moves with the step placement near
+    the peak: perturbing the oracle's own initial radius by 1e-15 moves it by up
+    to ~1e-5 (tests/test_oracle_windows.py).  It is held to 1e-6 for 98% of the
+    entries and 2e-4 for all."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from oracle import ckm1d_oracle as O
+
+    print("case: param study 8x8 grid, 12 periods, windows 4..11", flush=True)
+    w = workloads.param_study(side=8, periods=12.0, discard=4)
+    m = workloads.build_model(w)
+    ref = O.host_buffers_dict(m)
+    spec = O.spec_from_model(m)
+    out = O.solve_periodic(spec, ref, 1.0, 4, 8)
+    max_r, mean_x = m.solve_periodic(period=1.0, first_window=4, n_windows=8)
+    torch.cuda.synchronize()
+    mr, mx = max_r.cpu().numpy()[:, :, :2], mean_x.cpu().numpy()[:, :, :2]
+    rel = np.abs(mr - out["max_r"][:, :, :2]) / out["max_r"][:, :, :2]
+    assert np.mean(rel <= 1e-6) >= 0.98, np.mean(rel <= 1e-6)
+    assert rel.max() <= 2e-4, rel.max()
+    np.testing.assert_allclose(mx, out["mean_x"][:, :, :2], rtol=1e-6, atol=1e-12)
+    flags = m.runtime.device_buffers.status.status_flags.cpu().numpy()
+    np.testing.assert_array_equal(flags, ref["status_flags"])
+    st = m.runtime.device_buffers.state.actual_state.cpu().numpy()[:, :, :2]
+    want = ref["actual_state"][:, :, :2]
+    assert np.all(np.abs(st - want) <= RTOL * _scale(want))

@@ -1,0 +1,255 @@
+"""Pins of the oracle's CKM1D right-hand side and parameters.
+
+Checks are chosen so that a dropped term, a wrong sign or index or a
+transposed operand fails one of them:
+  * the dimensionless uncoupled RHS equals the textbook SI Keller--Miksis
+    equation (with the translational feedback term x'^2/4) and the
+    added-mass translational equation with primary Bjerknes and Levich drag,
+    evaluated from the material constants directly;
+  * the equilibrium invariant and the linear (Minnaert, surface-tension
+    corrected) stiffness;
+  * field identities (SW_A is SW_N shifted by a quarter wavelength; CONST has
+    no gradient or particle velocity);
+  * coupling: far-field decay orders, permutation equivariance, agreement of
+    the per-unit C form (ckm1d_def.cuh) with the NumPy matrix form
+    (CKM1D_reference.py) solved by numpy.linalg.solve, and the Richardson
+    solve of the reference agreeing with the direct solve;
+  * the collision event against a hand-built configuration.
+"""
+import numpy as np
+import pytest
+
+from oracle import ckm1d_numpy as NP
+from oracle import ckm1d_oracle as O
+
+
+def _model(R0, X0, PA=(1.0, 0.5), FR=(25.0, 50.0), field="SW_N", ref=25.0, mat=None):
+    from coupledbubble_control import workloads
+    from coupledbubble_control.models.model import water_25c
+
+    R0 = np.atleast_2d(np.asarray(R0, float))
+    n = R0.shape[1]
+    nk = len(PA)
+    w = workloads.Workload("t", n, R0, np.atleast_2d(np.asarray(X0, float)), np.asarray(PA, float)[:, None],
+                           np.asarray(FR, float)[:, None], np.zeros((nk, 1)), np.array([1.0]), ref_freq=ref,
+                           field=field, mat=mat or water_25c())
+    m = workloads.build_model(w)
+    return m, O.host_buffers_dict(m), O.spec_from_model(m)
+
+
+def _si_uncoupled(R, Rd, xd, p_a, dp_a_dt, grad_p, u_ac, R0, mat):
+    """Textbook SI accelerations (R'', x'') of a single bubble:
+    (1 - R'/c) R R'' + 3/2 (1 - R'/(3c)) R'^2 - x'^2/4
+        = (1 + R'/c)(p_L - p_inf)/rho + R/(rho c) d(p_L - p_inf)/dt,
+    p_L = (P0 - Pv + 2 s/R0)(R0/R)^(3k) + Pv - 2 s/R - 4 mu R'/R,  p_inf = P0 + p_a(t);
+    (2/3) pi rho R^3 x'' + 2 pi rho R^2 R' x' = -(4/3) pi R^3 grad p - 12 pi mu R (x' - u_ac)."""
+    rho, c, mu, sg, P0, Pv, k = mat.RHO, mat.CL, mat.VIS, mat.ST, mat.P0, mat.PV, mat.PE
+    G = P0 - Pv + 2 * sg / R0
+    pg = G * (R0 / R) ** (3 * k)
+    dpg = -3 * k * pg * Rd / R
+    pL_minus = pg + Pv - 2 * sg / R - 4 * mu * Rd / R - P0 - p_a
+    # d(pL - pinf)/dt without the R'' part of the viscous term (moved to the left side)
+    dpL_minus = dpg + 2 * sg * Rd / R ** 2 + 4 * mu * Rd ** 2 / R ** 2 - dp_a_dt
+    lhs_coef = (1 - Rd / c) * R + 4 * mu / (rho * c)
+    rhs = ((1 + Rd / c) * pL_minus / rho + R / (rho * c) * dpL_minus - 1.5 * (1 - Rd / (3 * c)) * Rd ** 2
+           + xd ** 2 / 4)
+    Rdd = rhs / lhs_coef
+    m_add = 2.0 / 3.0 * np.pi * rho * R ** 3
+    F = -(4.0 / 3.0) * np.pi * R ** 3 * grad_p - 12 * np.pi * mu * R * (xd - u_ac)
+    xdd = (F - 2 * np.pi * rho * R ** 2 * Rd * xd) / m_add
+    return Rdd, xdd
+
+
+@pytest.mark.parametrize("R0um,Rt,Ut,Vt,tau,x", [(44.0, 0.8, 0.3, 1e-4, 0.37, 0.1),
+                                                  (60.0, 1.7, -2.0, -3e-4, 1.2, 0.23),
+                                                  (12.0, 0.3, 5.0, 2e-3, 3.9, 0.41)])
+def test_uncoupled_rhs_matches_si_equations(R0um, Rt, Ut, Vt, tau, x):
+    from coupledbubble_control.models.model import water_25c
+
+    mat = water_25c()
+    m, bufs, spec = _model([[R0um]], [[x]], mat=mat)
+    f_ref = 25e3
+    lam = mat.CL / f_ref
+    R0 = R0um * 1e-6
+    y = np.array([Rt, x, Ut, Vt])
+    dy = O.rhs(spec, bufs, 0, tau, y)
+    # SI state
+    R, Rd, xd = Rt * R0, Ut * R0 * f_ref, Vt * mat.CL
+    t = tau / f_ref
+    p = pt = px = ux = 0.0
+    for A, f in ((1.0e5, 25e3), (0.5e5, 50e3)):
+        w, kw = 2 * np.pi * f, 2 * np.pi * f / mat.CL
+        X = x * lam
+        p += A * np.sin(kw * X) * np.sin(w * t)
+        pt += A * w * np.sin(kw * X) * np.cos(w * t)
+        px += A * kw * np.cos(kw * X) * np.sin(w * t)
+        ux += A / (mat.RHO * mat.CL) * np.cos(kw * X) * np.cos(w * t)
+    Rdd, xdd = _si_uncoupled(R, Rd, xd, p, pt, px, ux, R0, mat)
+    assert dy[0] == Ut and dy[1] == Vt
+    np.testing.assert_allclose(dy[2], Rdd / (R0 * f_ref ** 2), rtol=1e-11)
+    np.testing.assert_allclose(dy[3], xdd / (mat.CL * f_ref), rtol=1e-10)
+
+
+def test_equilibrium_invariant():
+    """R/R0 = 1, U = V = 0 and no drive: the bubble stays at rest."""
+    for R0um in (5.0, 44.0, 80.0):
+        m, bufs, spec = _model([[R0um, 30.0]], [[0.1, 0.3]], PA=(0.0, 0.0))
+        dy = O.rhs(spec, bufs, 0, 0.3, np.array([1.0, 1.0, 0.1, 0.3, 0.0, 0.0, 0.0, 0.0]))
+        scale = bufs["unit_params"][0, 0, 0]
+        assert np.all(np.abs(dy[4:]) <= 1e-13 * max(scale, 1.0))
+
+
+def test_minnaert_stiffness():
+    """Linearised about equilibrium: -dN/dR / D0 = omega_0^2/(f_ref^2) with
+    rho R0^2 omega_0^2 = 3 kappa (P0 - Pv + 2 s/R0) - 2 s/R0 and D0 = 1 + 4 mu/(rho c R0)."""
+    from coupledbubble_control.models.model import water_25c
+
+    mat = water_25c()
+    f_ref = 25e3
+    for R0um in (10.0, 44.0, 150.0):
+        m, bufs, spec = _model([[R0um]], [[0.2]], PA=(0.0, 0.0), mat=mat)
+        R0 = R0um * 1e-6
+        eps = 1e-6
+        dp = O.rhs(spec, bufs, 0, 0.0, np.array([1 + eps, 0.2, 0.0, 0.0]))[2]
+        dm = O.rhs(spec, bufs, 0, 0.0, np.array([1 - eps, 0.2, 0.0, 0.0]))[2]
+        stiff = -(dp - dm) / (2 * eps)
+        w0sq = (3 * mat.PE * (mat.P0 - mat.PV + 2 * mat.ST / R0) - 2 * mat.ST / R0) / (mat.RHO * R0 ** 2)
+        D0 = 1 + 4 * mat.VIS / (mat.RHO * mat.CL * R0)
+        np.testing.assert_allclose(stiff, w0sq / f_ref ** 2 / D0, rtol=1e-7)
+
+
+def test_sw_a_is_quarter_wave_shift_of_sw_n():
+    lam_k = 1.0  # f = f_ref: one reference wavelength
+    R0 = [[30.0, 50.0]]
+    X = np.array([0.11, 0.2])
+    m_n, b_n, s_n = _model(R0, [X + lam_k / 4], PA=(1.2,), FR=(25.0,), field="SW_N")
+    m_a, b_a, s_a = _model(R0, [X], PA=(1.2,), FR=(25.0,), field="SW_A")
+    y = np.array([0.9, 1.1, 0.0, 0.0, 0.4, -0.7, 1e-4, 2e-4])
+    ya, yn = y.copy(), y.copy()
+    ya[2:4] = X
+    yn[2:4] = X + lam_k / 4
+    np.testing.assert_allclose(O.rhs(s_a, b_a, 0, 0.77, ya)[4:], O.rhs(s_n, b_n, 0, 0.77, yn)[4:], rtol=1e-9)
+
+
+def test_const_field_has_no_translational_forcing():
+    m, bufs, spec = _model([[40.0]], [[0.2]], field="CONST")
+    dy = O.rhs(spec, bufs, 0, 0.3, np.array([1.3, 0.2, 0.5, 0.0]))
+    assert dy[3] == 0.0
+
+
+def _coupled_and_single(d, Rt=(0.9, 1.2), Ut=(0.4, -0.3), Vt=(0.0, 0.0)):
+    R0 = [[40.0, 55.0]]
+    m, bufs, spec = _model(R0, [[0.1, 0.1 + d]])
+    y = np.array([Rt[0], Rt[1], 0.1, 0.1 + d, Ut[0], Ut[1], Vt[0], Vt[1]])
+    dy2 = O.rhs(spec, bufs, 0, 0.4, y)
+    m1, b1, s1 = _model([[40.0]], [[0.1]])
+    dy1 = O.rhs(s1, b1, 0, 0.4, np.array([Rt[0], 0.1, Ut[0], Vt[0]]))
+    return dy2, dy1
+
+
+def test_far_field_decay_orders():
+    """Coupling corrections vanish as d grows: radial ~ 1/d (monopole), translational ~ 1/d^2."""
+    prev = None
+    for d in (10.0, 20.0, 40.0):
+        dy2, dy1 = _coupled_and_single(d)
+        dr = dy2[4] - dy1[2]
+        dt = dy2[6] - dy1[3]
+        if prev is not None:
+            assert prev[0] / dr == pytest.approx(2.0, rel=0.02)
+            assert prev[1] / dt == pytest.approx(4.0, rel=0.05)
+        prev = (dr, dt)
+    dy2, dy1 = _coupled_and_single(1e6)
+    np.testing.assert_allclose(dy2[4], dy1[2], rtol=1e-5)
+
+
+def test_permutation_equivariance():
+    """Relabelling the bubbles permutes the derivatives (3 bubbles, all terms active)."""
+    R0 = np.array([20.0, 45.0, 70.0])
+    X = np.array([0.1, 0.13, 0.2])
+    y = np.array([0.9, 1.2, 1.05, 0.1, 0.13, 0.2, 0.4, -0.3, 0.8, 1e-4, -2e-4, 3e-4])
+    m, b, s = _model([R0], [X])
+    d = O.rhs(s, b, 0, 0.4, y)
+    perm = np.array([2, 0, 1])
+    m2, b2, s2 = _model([R0[perm]], [X[perm]])
+    yp = y.reshape(4, 3)[:, perm].ravel()
+    dp = O.rhs(s2, b2, 0, 0.4, yp)
+    np.testing.assert_allclose(dp, d.reshape(4, 3)[:, perm].ravel(), rtol=1e-12, atol=1e-18)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 8])
+def test_per_unit_form_matches_matrix_form(n):
+    """C per-unit RHS (ckm1d_def.cuh form, LU solve) == NumPy matrix form
+    (CKM1D_reference.py form, numpy.linalg.solve)."""
+    rng = np.random.default_rng(n)
+    R0 = rng.uniform(10, 80, size=n)
+    X = np.sort(rng.uniform(0, 0.5, size=n))
+    m, b, s = _model([R0], [X])
+    up, gp, dp, mx, _ = NP.from_reference_layout(b, 0, n, 2)
+    for trial in range(3):
+        y = np.concatenate([rng.uniform(0.5, 1.5, n), X + rng.uniform(-1e-3, 1e-3, n),
+                            rng.uniform(-2, 2, n), rng.uniform(-1e-3, 1e-3, n)])
+        c = O.rhs(s, b, 0, 0.3 * trial, y)
+        ref = NP.rhs(0.3 * trial, y, up, gp, dp, mx, "SW_N")
+        np.testing.assert_allclose(c, ref, rtol=1e-11, atol=1e-14 * np.abs(ref).max())
+
+
+def test_implicit_matrix_matches_matrix_free_product():
+    """The assembled A (CKM1D_reference.py:672-693) equals the matrix-free _AxV (:310-331)."""
+    rng = np.random.default_rng(0)
+    n = 5
+    R0 = rng.uniform(10, 80, size=n)
+    X = np.sort(rng.uniform(0, 0.5, size=n))
+    m, b, s = _model([R0], [X])
+    up, gp, dp, mx, _ = NP.from_reference_layout(b, 0, n, 2)
+    y = np.concatenate([rng.uniform(0.5, 1.5, n), X, rng.uniform(-2, 2, n), rng.uniform(-1e-3, 1e-3, n)])
+    _, G, H, rd, sg = NP.coupling_parts(y, up, gp, dp, mx, 0.1, "SW_N")
+    A = NP.implicit_matrix(rd, sg, mx, G, H)
+    for _ in range(3):
+        v = rng.normal(size=2 * n)
+        np.testing.assert_allclose(A @ v, NP.matvec(rd, sg, mx, G, H, v), rtol=1e-13, atol=1e-15)
+
+
+def test_richardson_solve_of_reference_agrees_with_direct():
+    """The reference's relaxed Richardson sweep (ckm1d_def.py:302-404, relaxation 2/3,
+    tolerance 1e-9) converges to the direct solution within its tolerance."""
+    rng = np.random.default_rng(3)
+    n = 4
+    R0 = rng.uniform(10, 80, size=n)
+    X = np.sort(rng.uniform(0, 0.5, size=n))
+    m, b, s = _model([R0], [X])
+    s_r = O.OracleSpec(**{**s.__dict__, "linsolve": "richardson", "lin_relax": 2.0 / 3.0})
+    for _ in range(5):
+        y = np.concatenate([rng.uniform(0.5, 1.5, n), X, rng.uniform(-2, 2, n), rng.uniform(-1e-3, 1e-3, n)])
+        d1 = O.rhs(s, b, 0, 0.2, y)
+        d2 = O.rhs(s_r, b, 0, 0.2, y)
+        np.testing.assert_allclose(d2[2 * n:], d1[2 * n:], rtol=1e-7, atol=1e-9 * np.abs(d1[2 * n:]).max())
+
+
+def test_collision_event_hand_built():
+    from coupledbubble_control.models.model import water_25c
+
+    mat = water_25c()
+    lam = mat.CL / 25e3
+    m, b, s = _model([[30.0, 50.0, 20.0]], [[0.1, 0.15, 0.16]])
+    y = np.array([1.0, 2.0, 0.5, 0.1, 0.15, 0.16, 0, 0, 0, 0, 0, 0])
+    r = np.array([30.0, 100.0, 10.0]) * 1e-6 / lam
+    want = min(0.05 - 1.1 * (r[0] + r[1]), 0.01 - 1.1 * (r[1] + r[2]))
+    assert O.event(s, b, 0, y) == pytest.approx(want, rel=1e-13)
+    assert NP.collision_event(y, NP.from_reference_layout(b, 0, 3, 2)[0]) == pytest.approx(want, rel=1e-13)
+
+
+def test_parameter_builders_agree():
+    """Product builder (models/ckm1d.py:133-202 restated) == oracle builder
+    (CKM1D_reference.py:775-829 restated)."""
+    from coupledbubble_control.models.model import water_25c
+
+    mat = water_25c()
+    R0 = np.array([12.0, 44.0, 61.0])
+    m, b, s = _model([R0], [[0.1, 0.2, 0.3]], PA=(1.1, 0.3), FR=(25.0, 50.0))
+    nmat = NP.Mat(PV=mat.PV, RHO=mat.RHO, ST=mat.ST, VIS=mat.VIS, CL=mat.CL, P0=mat.P0, PE=mat.PE)
+    up, gp, dp, mx = NP.parameters(R0, [25.0, 50.0], [1.1, 0.3], [0.0, 0.0], 25.0, nmat)
+    up2, gp2, dp2, mx2, _ = NP.from_reference_layout(b, 0, 3, 2)
+    np.testing.assert_allclose(up2, up, rtol=1e-14)
+    np.testing.assert_allclose(gp2, gp, rtol=1e-15)
+    np.testing.assert_allclose(dp2, dp, rtol=1e-15)
+    np.testing.assert_allclose(mx2, mx, rtol=1e-14)
