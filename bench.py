@@ -55,9 +55,14 @@ def parse():
     p.add_argument("--periods-per-step", type=float, default=10.0)
     p.add_argument("--side", type=int, default=256, help="PA grid side (systems = side^2)")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
-    p.add_argument("--cpu-systems", type=int, default=512)
-    p.add_argument("--cpu-periods", type=float, default=2.0)
+    p.add_argument("--cpu-systems", type=int, default=2048)
+    p.add_argument("--cpu-periods", type=float, default=20.0)
     return p.parse_args()
+
+
+def log(msg: str) -> None:
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -67,6 +72,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    log("importing torch")
     import numpy as np
     import torch
 
@@ -98,12 +104,14 @@ def main():
         solve_periodic(m.runtime.kernel, db, w.period, w.first_window, max_r, mean_x)
 
     # warm-up on a discarded copy
+    log(f"building {ns} systems")
     mw, rw, xw = make()
     for _ in range(args.warmup):
         step(mw, rw, xw)
     torch.cuda.synchronize()
     del mw, rw, xw
 
+    log("warm-up done")
     m, max_r, mean_x = make()
     db = m.runtime.device_buffers
     stream = torch.cuda.current_stream()
@@ -118,6 +126,7 @@ def main():
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {wall:.3f} s")
     advanced = (db.state.actual_time - t_before).sum().item()
     attempts = db.status.total_steps.to(torch.int64).sum().item() - att_before
     kernel_ms = [a.elapsed_time(b) for a, b in ev]
@@ -136,9 +145,11 @@ def main():
         ms = W.build_model(sub)
         bufs = O.host_buffers_dict(ms)
         spec = O.spec_from_model(ms)
+        log(f"cpu baseline: {args.cpu_systems} systems x {args.cpu_periods:g} periods on {nthreads} threads")
         c0 = time.perf_counter()
         out = O.solve_periodic(spec, bufs, w.period, 0, 0)
         cwall = time.perf_counter() - c0
+        log(f"cpu baseline done in {cwall:.1f} s")
         cpu = {"value": float(np.sum(bufs["actual_time"]) / cwall), "unit": "periods/s", "cores": nthreads,
                "kind": "port",
                "sample": f"{args.cpu_systems} grid points x {args.cpu_periods:g} periods (same physics, "

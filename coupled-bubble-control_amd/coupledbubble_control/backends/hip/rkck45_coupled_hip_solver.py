@@ -119,8 +119,10 @@ def launch(kernel, d_buffers, kernel_steps: int, stream=None) -> None:
 
 
 def solve_periodic(kernel, d_buffers, period: float, first_window: int, max_r: torch.Tensor,
-                   mean_x: torch.Tensor, max_steps: int = 1 << 40, stream=None) -> None:
-    """Integrate to time_end in windows of ``period`` recording window statistics (asynchronous)."""
+                   mean_x: torch.Tensor, max_steps: int = 1_000_000, stream=None) -> None:
+    """One launch integrating to time_end in windows of ``period`` and recording window
+    statistics (asynchronous).  At most ``max_steps`` attempts per system: a system that
+    runs out stays ACTIVE with its open window saved, and the next call resumes it."""
     lib = _abi.lib()
     _check_shapes(d_buffers, kernel.model_spec, kernel.execution_spec)
     _check_trig_range(d_buffers, kernel.model_spec)

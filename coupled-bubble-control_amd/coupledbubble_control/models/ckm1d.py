@@ -175,12 +175,14 @@ class CKM1D(Model):
         return partial(solver, kernel=kernel)
 
     def solve_periodic(self, period: float = 1.0, first_window: int = 0, n_windows: Optional[int] = None,
-                       max_steps: int = 1 << 40, stream=None):
+                       max_steps: int = 1_000_000, stream=None, max_launches: int = 1000):
         """Integrate every system to its time_end in windows of ``period`` and return
         (max_r, mean_x) shaped (n_windows, NS, TILE): the largest R/R0 over the
         accepted states of each window and the trapezoidal time-mean of x/lambda.
-        Windows with index < first_window are skipped.  Extension of the reference,
-        whose parameter study derives such outputs from dense output."""
+        Windows with index < first_window are skipped.  Launches of at most
+        ``max_steps`` attempts per system repeat while any system is ACTIVE (at most
+        ``max_launches``).  Extension of the reference, whose parameter study derives
+        such outputs from dense output."""
         import torch
 
         from ..backends.hip.rkck45_coupled_hip_solver import solve_periodic
@@ -196,5 +198,8 @@ class CKM1D(Model):
         dev = db.state.actual_state.device
         max_r = torch.zeros((n_windows, es.ns, es.tile), dtype=torch.float64, device=dev)
         mean_x = torch.zeros((n_windows, es.ns, es.tile), dtype=torch.float64, device=dev)
-        solve_periodic(self.runtime.kernel, db, period, first_window, max_r, mean_x, max_steps, stream)
+        for _ in range(max_launches):
+            solve_periodic(self.runtime.kernel, db, period, first_window, max_r, mean_x, max_steps, stream)
+            if not bool((db.status.status_flags == 0).any().item()):
+                break
         return max_r, mean_x

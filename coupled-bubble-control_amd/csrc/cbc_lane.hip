@@ -260,7 +260,9 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
         }
     }
 
-    // period windows: a window end is an intermediate end time of the clamp
+    // period windows: a window end is an intermediate end time of the clamp.
+    // A launch that starts inside a window resumes its partial statistics,
+    // which the previous launch left in max_r / mean_x.
     const bool win = a.period > 0.0;
     int wcur = 0;
     double wend = t_end;
@@ -268,10 +270,17 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
     if (win) {
         wcur = (int)floor(t / a.period + 1e-12);
         wend = fmin((wcur + 1) * a.period, t_end);
+        const int widx = wcur - a.first_window;
+        const bool mid = (t - wcur * a.period) > 1e-9 * a.period;
+        const bool stored = widx >= 0 && widx < a.n_windows;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             wmax[i] = y[i];
             wsum[i] = 0.0;
+            if (mid && stored) {
+                wmax[i] = a.max_r[((size_t)widx * NS + s) * TILE + i];
+                wsum[i] = a.mean_x[((size_t)widx * NS + s) * TILE + i] * a.period;
+            }
         }
     }
 
@@ -320,6 +329,10 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
         // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
         const bool crossing = g_old * g_new < 0.0;
         if (crossing) {
+            if (h <= a.min_step) {  // cannot shrink further: no progress possible
+                flag = kStepSizeLimit;
+                break;
+            }
             const double aold = fabs(g_old);
             const double ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
             h = h * ratio;
@@ -342,6 +355,13 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
                 break;
             }
             const bool ok = ratio >= 1.0;
+            // step size below the time resolution: an accepted step would not
+            // advance t, a rejected one cannot shrink (the reference clamps h at
+            // MIN_STEP and spins until its step budget runs out)
+            if (ok ? (t + h == t) : (h <= a.min_step)) {
+                flag = kStepSizeLimit;
+                break;
+            }
             const double fac = step_factor(ok, ratio);
             if (ok) {
                 if (win) {
@@ -398,6 +418,17 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
         if (fabs(g_old) < a.etol) {
             flag = kEventTerminal;
             break;
+        }
+    }
+
+    if (win && flag != kSuccess) {
+        const int widx = wcur - a.first_window;
+        if ((t - wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax[i];
+                a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum[i] / a.period;
+            }
         }
     }
 

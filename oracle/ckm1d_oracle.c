@@ -330,7 +330,7 @@ double or_fac_bad(double ratio) { return fmin(0.5, fmax(0.1, 0.9 * pow(ratio, 0.
 typedef struct {
     double t, t_end, t_begin, h, event;
     int flag;
-    int64_t accepted, rejected;
+    int64_t accepted, rejected, attempts;
     int dense_index;
 } or_run;
 
@@ -374,6 +374,20 @@ static void or_window_update(const or_system *S, const or_period *P, double t0, 
     }
 }
 
+/* Leave the statistics of a window that is still open (budget exhausted or the
+   system stopped) so that the next call can resume it. */
+static void or_window_flush(const or_period *P, int wcur, double t, int n, const double *wmax, const double *wsum)
+{
+    if (!P || P->period <= 0.0) return;
+    int widx = wcur - P->first_window;
+    if ((t - wcur * P->period) > 1e-9 * P->period && widx >= 0 && widx < P->n_windows) {
+        for (int i = 0; i < n; i++) {
+            P->max_r[widx * n + i] = wmax[i];
+            P->mean_x[widx * n + i] = wsum[i] / P->period;
+        }
+    }
+}
+
 int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int kernel_steps,
                int ndo, double *dtime, double *dstate, const or_period *P)
 {
@@ -399,7 +413,15 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
     if (P && P->period > 0.0) {
         wcur = (int)floor(st->t / P->period + 1e-12);
         wend = fmin((wcur + 1) * P->period, st->t_end);
-        for (int i = 0; i < n; i++) { wmax[i] = y[i]; wsum[i] = 0.0; }
+        int widx = wcur - P->first_window;
+        int mid = (st->t - wcur * P->period) > 1e-9 * P->period;
+        for (int i = 0; i < n; i++) {
+            wmax[i] = y[i]; wsum[i] = 0.0;
+            if (mid && widx >= 0 && widx < P->n_windows) {   /* resume a partial window */
+                wmax[i] = P->max_r[widx * n + i];
+                wsum[i] = P->mean_x[widx * n + i] * P->period;
+            }
+        }
     }
 
     double t = st->t, h = st->h;
@@ -408,6 +430,7 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
     int attempts = 0;
     for (int step = 0; step < kernel_steps; step++) {
         attempts++;
+        st->attempts++;
         /* clamp: stepper.py:366-373 */
         if (h < T->min_step) h = T->min_step;
         if (h > T->max_step) h = T->max_step;
@@ -440,6 +463,12 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
         for (int c = 0; c < d; c++) if (!isfinite(ynew[c])) nan_state = 1;
 
         if (crossing) {
+            if (h <= T->min_step) {       /* no progress possible */
+                st->flag = OR_STEP_SIZE_LIMIT;
+                st->t = t; st->h = h; st->event = g_old;
+                or_window_flush(P, wcur, t, n, wmax, wsum);
+                return attempts;
+            }
             h = h * ev_ratio;
             st->rejected++;
         } else {
@@ -459,6 +488,15 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
             if (nan_state) {
                 st->flag = OR_NAN;
                 st->t = t; st->h = h; st->event = g_old;
+                or_window_flush(P, wcur, t, n, wmax, wsum);
+                return attempts;
+            }
+            /* step size below the time resolution (the reference clamps at MIN_STEP
+               and spins until its step budget runs out) */
+            if (ok ? (t + h == t) : (h <= T->min_step)) {
+                st->flag = OR_STEP_SIZE_LIMIT;
+                st->t = t; st->h = h; st->event = g_old;
+                or_window_flush(P, wcur, t, n, wmax, wsum);
                 return attempts;
             }
             /* controller: stepper.py:585-618 */
@@ -510,11 +548,13 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
         if (fabs(g_old) < T->etol) {
             st->flag = OR_EVENT_TERMINAL;
             st->h = h; st->event = g_old;
+            or_window_flush(P, wcur, t, n, wmax, wsum);
             return attempts;
         }
     }
     st->h = h;
     st->event = g_old;
+    or_window_flush(P, wcur, t, n, wmax, wsum);
     return attempts;
 }
 
@@ -579,7 +619,7 @@ int or_solve_reference_layout(const or_config *C, int kernel_steps, int64_t max_
         or_run st;
         st.t = actual_time[s]; st.t_end = time_end[s]; st.t_begin = time_begin[s];
         st.h = time_step[s]; st.event = actual_event[s]; st.flag = 0;
-        st.accepted = 0; st.rejected = 0;
+        st.accepted = 0; st.rejected = 0; st.attempts = 0;
         st.dense_index = NDO > 0 ? dense_index[s] : 0;
         double *dt = NULL, *ds = NULL;
         if (NDO > 0) {
@@ -601,7 +641,7 @@ int or_solve_reference_layout(const or_config *C, int kernel_steps, int64_t max_
             for (int u = 0; u < n; u++) actual_state[(size_t)j * NS * TILE + (size_t)s * TILE + u] = y[j * n + u];
         actual_time[s] = st.t; time_step[s] = st.h; actual_event[s] = st.event;
         status_flags[s] = st.flag;
-        if (total_steps) total_steps[s] = st.accepted + st.rejected;
+        if (total_steps) total_steps[s] = st.attempts;
         if (rejected_steps) rejected_steps[s] = st.rejected;
         if (NDO > 0) {
             dense_index[s] = st.dense_index;
@@ -647,10 +687,14 @@ int or_solve_periodic(const or_config *C, int64_t max_attempts, double period, i
         or_run st;
         st.t = actual_time[s]; st.t_end = time_end[s]; st.t_begin = 0.0;
         st.h = time_step[s]; st.event = actual_event[s]; st.flag = 0;
-        st.accepted = 0; st.rejected = 0; st.dense_index = 0;
+        st.accepted = 0; st.rejected = 0; st.attempts = 0; st.dense_index = 0;
         double *mr = (double *)malloc(sizeof(double) * (size_t)(n_windows > 0 ? n_windows : 1) * n);
         double *mx = (double *)malloc(sizeof(double) * (size_t)(n_windows > 0 ? n_windows : 1) * n);
-        for (int q = 0; q < n_windows * n; q++) { mr[q] = 0.0; mx[q] = 0.0; }
+        for (int q = 0; q < n_windows; q++)       /* caller's arrays: resumes open windows */
+            for (int u = 0; u < n; u++) {
+                mr[q * n + u] = max_r[((size_t)q * NS + s) * TILE + u];
+                mx[q * n + u] = mean_x[((size_t)q * NS + s) * TILE + u];
+            }
         or_period P = {period, first_window, n_windows, mr, mx};
         /* windows force a restart of the attempt loop at each boundary */
         int64_t used = 0;
@@ -663,7 +707,7 @@ int or_solve_periodic(const or_config *C, int64_t max_attempts, double period, i
             for (int u = 0; u < n; u++) actual_state[(size_t)j * NS * TILE + (size_t)s * TILE + u] = y[j * n + u];
         actual_time[s] = st.t; time_step[s] = st.h; actual_event[s] = st.event;
         status_flags[s] = st.flag;
-        if (total_steps) total_steps[s] = st.accepted + st.rejected;
+        if (total_steps) total_steps[s] = st.attempts;
         if (rejected_steps) rejected_steps[s] = st.rejected;
         for (int q = 0; q < n_windows; q++)
             for (int u = 0; u < n; u++) {

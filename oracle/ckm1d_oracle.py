@@ -169,6 +169,25 @@ def solve_periodic(spec: OracleSpec, bufs: dict, period: float, first_window: in
     return {"total_steps": total, "rejected_steps": rej, "max_r": max_r, "mean_x": mean_x}
 
 
+def solve_periodic_into(spec: OracleSpec, bufs: dict, period: float, first_window: int, n_windows: int,
+                        max_r: np.ndarray, mean_x: np.ndarray, max_attempts: int) -> dict:
+    """Like solve_periodic but accumulating into caller-owned window arrays, so that
+    budget-limited calls resume open windows."""
+    L = lib()
+    cfg = spec.c()
+    total = np.zeros(spec.ns, dtype=np.int64)
+    rej = np.zeros(spec.ns, dtype=np.int64)
+    assert max_r.shape == (n_windows, spec.ns, spec.tile) and max_r.flags.c_contiguous
+    assert mean_x.shape == (n_windows, spec.ns, spec.tile) and mean_x.flags.c_contiguous
+    L.or_solve_periodic(
+        ctypes.byref(cfg), int(max_attempts), float(period), int(first_window), int(n_windows),
+        _p(bufs["actual_time"]), _p(bufs["time_end"]), _p(bufs["time_step"]), _p(bufs["actual_state"]),
+        _p(bufs["unit_params"]), _p(bufs["system_params"]), _p(bufs["control_params"]),
+        _p(bufs["coupling_matrices"]), _p(bufs["actual_event"]), _p(bufs["status_flags"], _pi),
+        _p(total, _pl), _p(rej, _pl), _p(max_r), _p(mean_x), 0, spec.ns)
+    return {"total_steps": total, "rejected_steps": rej}
+
+
 def rhs(spec: OracleSpec, bufs: dict, s: int, t: float, y: np.ndarray) -> np.ndarray:
     """Coupled RHS of system s at (t, y), y ordered [R(n), x(n), U(n), V(n)]."""
     L = lib()

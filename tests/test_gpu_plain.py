@@ -243,3 +243,54 @@ def test_period_windows(device):
     st = m.runtime.device_buffers.state.actual_state.cpu().numpy()[:, :, :2]
     want = ref["actual_state"][:, :, :2]
     assert np.all(np.abs(st - want) <= RTOL * _scale(want))
+
+
+@pytest.mark.parametrize("case", ["rejected_at_min_step", "no_progress"])
+def test_step_size_limit_forced(device, case):
+    """Both STEP_SIZE_LIMIT (-2) branches, forced from the host: a fixed step far too large
+    for the tolerance is rejected at MIN_STEP; a fixed 1e-18 step at t = 10 does not
+    advance t.  (The physical trigger, a wall speed reaching c where the Keller--Miksis
+    denominator vanishes, occurs at a chaotic point of long strongly driven runs; the
+    oracle test test_keller_miksis_singularity_is_flagged pins it on the CPU.)"""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.models.specs import SolverSpec
+    from oracle import ckm1d_oracle as O
+
+    print(f"case: step-size limit {case}", flush=True)
+    w = workloads.plain_cpu(seed=13, ns=64)
+    if case == "rejected_at_min_step":
+        spec = SolverSpec(atol=1e-14, rtol=1e-14, min_step=0.01, max_step=0.01)
+    else:
+        spec = SolverSpec(atol=1e-9, rtol=1e-9, min_step=1e-18, max_step=1e-18)
+    m = workloads.build_model(w, solver_spec=spec)
+    if case == "no_progress":
+        m.runtime.host_buffers.state.actual_time[:] = 10.0
+        m.runtime.host_buffers.state.time_end[:] = 11.0
+    ref = O.host_buffers_dict(m)
+    O.solve(O.spec_from_model(m), ref, kernel_steps=100, max_steps=100)
+    m.solve(max_steps=100, kernel_steps=100, sync_to_host=True)
+    hb = m.runtime.host_buffers
+    assert np.all(ref["status_flags"] == -2)
+    np.testing.assert_array_equal(hb.status.status_flags, ref["status_flags"])
+    np.testing.assert_array_equal(hb.status.total_steps, 1)
+    np.testing.assert_array_equal(hb.state.actual_time, ref["actual_time"])
+
+
+def test_period_windows_resume_across_launches(device):
+    """Budget-limited launches (777 attempts each) reproduce one long launch bit for bit."""
+    import torch
+
+    from coupledbubble_control import workloads
+
+    print("case: window resume, 16 systems x 5 periods, budget 777", flush=True)
+    w = workloads.param_study(side=4, periods=5.0, discard=1)
+    m1 = workloads.build_model(w)
+    r1, x1 = m1.solve_periodic(period=1.0, first_window=1, n_windows=4, max_steps=1_000_000, max_launches=1)
+    m2 = workloads.build_model(w)
+    r2, x2 = m2.solve_periodic(period=1.0, first_window=1, n_windows=4, max_steps=777, max_launches=400)
+    torch.cuda.synchronize()
+    assert int((m2.runtime.device_buffers.status.total_steps > 777).sum().item()) > 0
+    for a, b in ((r1, r2), (x1, x2)):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    np.testing.assert_array_equal(m1.runtime.device_buffers.state.actual_state.cpu().numpy(),
+                                  m2.runtime.device_buffers.state.actual_state.cpu().numpy())

@@ -69,3 +69,46 @@ def test_window_max_is_step_placement_sensitive():
     assert d_mean.max() < 1e-9
     assert d_max.max() > 1e-6
     assert d_max.max() < 2e-4
+
+
+def test_windows_resume_across_budget_limited_calls():
+    """Budget-limited calls (the system stays ACTIVE with its open window saved) give the
+    same statistics as one unlimited call."""
+    w, m, spec = _grid(side=3, periods=5.0, discard=1)
+    a, b = O.host_buffers_dict(m), O.host_buffers_dict(m)
+    one = O.solve_periodic(spec, a, 1.0, 1, 4)
+    mr = np.zeros_like(one["max_r"])
+    mx = np.zeros_like(one["mean_x"])
+    calls = 0
+    while True:
+        part = O.solve_periodic_into(spec, b, 1.0, 1, 4, mr, mx, max_attempts=777)
+        calls += 1
+        if np.all(b["status_flags"] != 0) or calls > 500:
+            break
+    assert calls > 5
+    np.testing.assert_array_equal(b["status_flags"], a["status_flags"])
+    np.testing.assert_array_equal(b["actual_state"], a["actual_state"])
+    np.testing.assert_array_equal(mr, one["max_r"])
+    np.testing.assert_allclose(mx, one["mean_x"], rtol=1e-14)
+
+
+def test_keller_miksis_singularity_is_flagged():
+    """At PA = (1.42, 2.0) bar the 44 um bubble's wall speed reaches the sound speed
+    (U -> lambda f / R0 = c/(R0 f)), the Keller--Miksis denominator vanishes and the
+    step size collapses below the time resolution.  The stepper reports
+    STEP_SIZE_LIMIT (-2) within a bounded number of attempts instead of spinning."""
+    from coupledbubble_control import workloads as W
+
+    w = W.param_study(side=32, periods=80.0, discard=0)
+    s = 735                                   # PA = (1.4194, 2.0) bar on the 32 x 32 grid
+    sub = w.subset([s])
+    m = W.build_model(sub)
+    b = O.host_buffers_dict(m)
+    spec = O.spec_from_model(m)
+    out = O.solve_periodic(spec, b, 1.0, 0, 80, max_attempts=3_000_000)
+    assert b["status_flags"][0] == -2
+    assert out["total_steps"][0] < 3_000_000
+    U = b["actual_state"][2, 0, 0]
+    c_over_R0f = 1497.0 / (44e-6 * 25e3)
+    assert 0.8 * c_over_R0f < abs(U) < 1.2 * c_over_R0f
+    assert 20.0 < b["actual_time"][0] < 80.0
