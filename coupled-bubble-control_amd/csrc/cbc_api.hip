@@ -87,16 +87,28 @@ int32_t cbc_plan_create(const cbc_model_spec* model, const cbc_exec_spec* exec, 
         if (solver->linsolve != CBC_LIN_DIRECT)
             return fail(CBC_ERR_UNSUPPORTED, "only the direct coupling solve (CBC_LIN_DIRECT) is built");
 
+        // AUTO: whole system in one lane while it fits in VGPRs (N <= 2), else one
+        // lane per bubble with the coupling solve staged in LDS
         int mapping = exec->mapping;
-        if (mapping == CBC_MAP_AUTO) mapping = CBC_MAP_LANE;
-        if (mapping != CBC_MAP_LANE)
-            return fail(CBC_ERR_UNSUPPORTED, "mapping %d is not built for ups=%d", mapping, exec->ups);
-        if (!cbc::lane_supported(exec->ups, model->nk, model->field))
-            return fail(CBC_ERR_UNSUPPORTED, "lane kernel not instantiated for ups=%d nk=%d field=%d", exec->ups,
-                        model->nk, model->field);
-        int block = exec->block_size > 0 ? exec->block_size : 64;
-        if (block > 256 || block % 64 != 0)
-            return fail(CBC_ERR_INVALID, "block_size must be a multiple of 64 in [64, 256], got %d", block);
+        if (mapping == CBC_MAP_AUTO) mapping = (exec->ups <= 2) ? CBC_MAP_LANE : CBC_MAP_GROUP;
+        if (mapping == CBC_MAP_GROUP && exec->ups == 1) mapping = CBC_MAP_LANE;
+        int block = 64;
+        if (mapping == CBC_MAP_LANE) {
+            if (!cbc::lane_supported(exec->ups, model->nk, model->field))
+                return fail(CBC_ERR_UNSUPPORTED, "lane kernel not instantiated for ups=%d nk=%d field=%d",
+                            exec->ups, model->nk, model->field);
+            block = exec->block_size > 0 ? exec->block_size : 64;
+            if (block > 256 || block % 64 != 0)
+                return fail(CBC_ERR_INVALID, "block_size must be a multiple of 64 in [64, 256], got %d", block);
+        } else if (mapping == CBC_MAP_GROUP) {
+            if (!cbc::group_supported(exec->ups, model->nk, model->field))
+                return fail(CBC_ERR_UNSUPPORTED, "group kernel not instantiated for ups=%d nk=%d field=%d",
+                            exec->ups, model->nk, model->field);
+            if (exec->block_size > 0 && exec->block_size != 64)
+                return fail(CBC_ERR_INVALID, "the group mapping runs one wave (64 threads) per block");
+        } else {
+            return fail(CBC_ERR_INVALID, "unknown mapping %d", mapping);
+        }
 
         cbc_plan* p = new (std::nothrow) cbc_plan();
         if (!p) return fail(CBC_ERR_NOMEM, "cbc_plan_create: out of host memory");
@@ -132,12 +144,15 @@ int32_t cbc_plan_query(const cbc_plan* plan, cbc_plan_info* out)
 {
     if (!plan || !out) return fail(CBC_ERR_INVALID, "cbc_plan_query: null argument");
     cbc::KernelAttr at{};
-    hipError_t e = cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+    const bool lane = plan->mapping == CBC_MAP_LANE;
+    hipError_t e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
+                        : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
+    const int spb = lane ? plan->block : cbc::group_systems_per_block(plan->exec.ups);
     out->mapping = plan->mapping;
     out->block_size = plan->block;
-    out->grid_size = (plan->exec.ns + plan->block - 1) / plan->block;
-    out->systems_per_block = plan->block;
+    out->grid_size = (plan->exec.ns + spb - 1) / spb;
+    out->systems_per_block = spb;
     out->lds_bytes = at.lds_bytes;
     out->num_regs = at.num_regs;
     out->local_bytes = at.local_bytes;
@@ -151,6 +166,11 @@ void cbc_plan_destroy(cbc_plan* plan)
     if (plan->d_active) (void)hipFree(plan->d_active);
     if (plan->h_active) (void)hipHostFree(plan->h_active);
     delete plan;
+}
+
+static hipError_t launch_kernel(const cbc_plan* p, const cbc::LaunchArgs& a, hipStream_t st)
+{
+    return p->mapping == CBC_MAP_LANE ? cbc::lane_launch(a, p->block, st) : cbc::group_launch(a, st);
 }
 
 static int32_t fill_args(const cbc_plan* p, const cbc_buffers* b, cbc::LaunchArgs& a)
@@ -203,8 +223,8 @@ int32_t cbc_launch(cbc_plan* plan, const cbc_buffers* buf, int32_t kernel_steps,
     if (rc != CBC_OK) return rc;
     a.kernel_steps = kernel_steps;
     a.active_count = nullptr;
-    hipError_t e = cbc::lane_launch(a, plan->block, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e, "lane kernel launch");
+    hipError_t e = launch_kernel(plan, a, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return CBC_OK;
 }
 
@@ -226,8 +246,8 @@ int32_t cbc_solve(cbc_plan* plan, const cbc_buffers* buf, int32_t kernel_steps, 
         auto t0 = std::chrono::steady_clock::now();
         hipError_t e = hipMemsetAsync(plan->d_active, 0, sizeof(uint32_t), st);
         if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(active counter)");
-        e = cbc::lane_launch(a, plan->block, st);
-        if (e != hipSuccess) return hip_fail(e, "lane kernel launch");
+        e = launch_kernel(plan, a, st);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
         e = hipMemcpyAsync(plan->h_active, plan->d_active, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(active counter)");
         e = hipStreamSynchronize(st);
@@ -262,8 +282,8 @@ int32_t cbc_solve_periodic(cbc_plan* plan, const cbc_buffers* buf, const cbc_per
     a.max_r = period->max_r;
     a.mean_x = period->mean_x;
     a.active_count = nullptr;
-    hipError_t e = cbc::lane_launch(a, plan->block, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e, "lane kernel launch (periodic)");
+    hipError_t e = launch_kernel(plan, a, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch (periodic)");
     return CBC_OK;
 }
 

@@ -1,0 +1,224 @@
+// cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
+// one lane per bubble, 64/TILE systems per wave (one wave per workgroup).
+//
+// This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
+// with a different coupling solve: instead of the block-synchronous Richardson
+// sweep (ckm1d_def.cuh:252-376), lane i owns rows i (radial) and N+i
+// (translational) of the 2N x (2N+1) augmented system in VGPRs and the group
+// runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
+// the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
+// butterfly argmax across the group; its owner stages it in LDS (double
+// buffered, one barrier per column) and every lane eliminates that column from
+// its rows.  Bubble data needed by the coupling sums (x, H0..H6) and the
+// coupling constants mx are staged in LDS per system.  All lanes of a group
+// reduce the error ratio and the collision gap to identical values, so the
+// step controller runs redundantly and stays consistent without a broadcast.
+#pragma once
+
+#include "cbc_device.h"
+#include "cbc_kernels.h"
+
+namespace cbc {
+
+constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2); }
+
+template <int N>
+struct GroupShape {
+    static constexpr int G = pow2_ceil(N);   // lanes per system
+    static constexpr int S = 64 / G;         // systems per wave
+    static constexpr int n = 2 * N;          // coupling unknowns
+};
+
+template <int N>
+struct GroupLds {
+    static constexpr int G = GroupShape<N>::G;
+    static constexpr int n = 2 * N;
+    double mx[3][N][N];        // mx[c][i][j]: effect of bubble j on bubble i
+    double xh[8][G];           // x, H0..H6 of every bubble
+    double ev[2][G];           // collision radius and position
+    double piv[2][n + 1];      // pivot row, double buffered
+    double sol[n];             // solved accelerations
+};
+
+// Butterfly reductions inside a group of G lanes (every lane gets the result).
+template <int G>
+__device__ __forceinline__ double group_min(double v)
+{
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v = fmin(v, __shfl_xor(v, m, G));
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool v)
+{
+    int x = v ? 1 : 0;
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) x |= __shfl_xor(x, m, G);
+    return x != 0;
+}
+
+// argmax of (value, id) pairs; ties resolved to the lower id
+template <int G>
+__device__ __forceinline__ void group_argmax(double& v, int& id)
+{
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        const double ov = __shfl_xor(v, m, G);
+        const int oid = __shfl_xor(id, m, G);
+        const bool take = (ov > v) || (ov == v && oid < id);
+        v = take ? ov : v;
+        id = take ? oid : id;
+    }
+}
+
+// Per-lane constants of one bubble and its system.
+template <int NK>
+struct GroupParams {
+    Unit up;
+    double rcol;
+    double sp0, sp3;
+    Drive<NK> d;
+};
+
+// Coupled RHS for the lane's bubble i.  y, dy = (R, x, U, V) of bubble i.
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>& L, int i, bool vu, double t,
+                                          const double (&y)[4], double (&dy)[4])
+{
+    constexpr int n = 2 * N;
+    double st[NK], ct[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) sincos_fast(P.d.wt[k] * t + P.d.ph[k], &st[k], &ct[k]);
+    double pa, pat, pax, uax;
+    field_at<NK, FIELD>(P.d, st, ct, y[1], pa, pat, pax, uax);
+    BubbleOut o;
+    bubble_rhs(P.up, P.sp0, P.sp3, y[0], y[2], y[3], pa, pat, pax, uax, o);
+    dy[0] = y[2];
+    dy[1] = y[3];
+    if constexpr (N == 1) {
+        dy[2] = o.dU;
+        dy[3] = o.dV;
+        return;
+    } else {
+        if (vu) {
+            L.xh[0][i] = y[1];
+            L.xh[1][i] = o.H0;
+            L.xh[2][i] = o.H1;
+            L.xh[3][i] = o.H2;
+            L.xh[4][i] = o.H3;
+            L.xh[5][i] = o.H4;
+            L.xh[6][i] = o.H5;
+            L.xh[7][i] = o.H6;
+        }
+        __syncthreads();
+
+        // explicit coupling sums and the two matrix rows owned by this lane
+        double ar[n + 1], at[n + 1];
+        double rg0 = 0.0, rg1 = 0.0, tng = 0.0, tg2 = 0.0, tg3 = 0.0;
+        const int ii = vu ? i : 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const double xj = L.xh[0][j];
+            const double delta = y[1] - xj;
+            const double s = sgn(delta);
+            const double r = rcp_fast(fmax(fabs(delta), 1e-30));
+            const double r2 = r * r;
+            const double sr2 = r2 * s;
+            const double r3 = r2 * r;
+            const double m0 = L.mx[0][ii][j], m1 = L.mx[1][ii][j], m2 = L.mx[2][ii][j];
+            const double H0 = L.xh[1][j], H1 = L.xh[2][j], H2 = L.xh[3][j], H3 = L.xh[4][j], H4 = L.xh[5][j];
+            const double H5 = L.xh[6][j], H6 = L.xh[7][j];
+            const bool self = (j == i);
+            if (!self) {
+                rg0 += m0 * (-2.0 * r * H0 - 2.5 * sr2 * H2 - 1.0 * r3 * H4);
+                rg1 += m0 * (-0.5 * sr2 * H1 - 0.5 * r3 * H3);
+                tng += m1 * (2.0 * sr2 * H0 + 5.0 * r3 * H2);
+                tg2 += m1 * (sr2 * H1 + r3 * H3);
+                tg3 += m2 * (sr2 * H1 + r3 * H3);
+            }
+            ar[j] = self ? 1.0 : o.G0 * m0 * r * H5;
+            ar[N + j] = self ? 0.0 : 0.5 * o.G0 * m0 * sr2 * H6;
+            at[j] = self ? 0.0 : -m1 * sr2 * H5;
+            at[N + j] = self ? 1.0 : -m1 * r3 * H6;
+        }
+        ar[n] = o.dU + (rg0 * o.G0 + rg1 * o.G1);
+        at[n] = o.dV + (tng + tg2 * o.G2 + tg3 * o.G3);
+
+        // Gauss--Jordan with partial pivoting across the group
+        bool used_r = !vu, used_t = !vu;
+        int col_r = 0, col_t = 0;
+        double inv_r = 0.0, inv_t = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            const double vr = used_r ? -1.0 : fabs(ar[k]);
+            const double vt = used_t ? -1.0 : fabs(at[k]);
+            double v = vr;
+            int id = i;
+            if (vt > vr) {
+                v = vt;
+                id = N + i;
+            }
+            group_argmax<GroupShape<N>::G>(v, id);
+            const bool own_r = vu && id == i;
+            const bool own_t = vu && id == N + i;
+            double* pv = L.piv[k & 1];
+            if (own_r) {
+#pragma unroll
+                for (int j = k; j <= n; ++j) pv[j] = ar[j];
+                used_r = true;
+                col_r = k;
+                inv_r = rcp_fast(ar[k]);
+            }
+            if (own_t) {
+#pragma unroll
+                for (int j = k; j <= n; ++j) pv[j] = at[j];
+                used_t = true;
+                col_t = k;
+                inv_t = rcp_fast(at[k]);
+            }
+            __syncthreads();
+            const double inv = rcp_fast(pv[k]);
+            const double fr = own_r ? 0.0 : ar[k] * inv;
+            const double ft = own_t ? 0.0 : at[k] * inv;
+#pragma unroll
+            for (int j = k + 1; j <= n; ++j) {
+                const double p = pv[j];
+                ar[j] -= fr * p;
+                at[j] -= ft * p;
+            }
+        }
+        if (vu) {
+            L.sol[col_r] = ar[n] * inv_r;
+            L.sol[col_t] = at[n] * inv_t;
+        }
+        __syncthreads();
+        dy[2] = L.sol[ii];
+        dy[3] = L.sol[N + ii];
+    }
+}
+
+// Collision gap of the system, identical on every lane of the group.
+template <int N>
+__device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, double rcol, const double (&y)[4],
+                                              double colf)
+{
+    if constexpr (N == 1) {
+        return 1e30;
+    } else {
+        if (vu) {
+            L.ev[0][i] = y[0] * rcol;
+            L.ev[1][i] = y[1];
+        }
+        __syncthreads();
+        double gap = 1e30;
+        if (vu && i + 1 < N) {
+            const double ri = L.ev[0][i], rj = L.ev[0][i + 1];
+            gap = fabs(L.ev[1][i] - L.ev[1][i + 1]) - (ri + rj) * colf;
+        }
+        __syncthreads();  // ev is rewritten by the next call
+        return group_min<GroupShape<N>::G>(gap);
+    }
+}
+
+}  // namespace cbc

@@ -1,0 +1,60 @@
+// Dispatch of the group-mapped kernels (cbc_group_kernel.h); the kernels are
+// instantiated per bubble count in cbc_group_n*.hip so they compile in parallel.
+#include "cbc_kernels.h"
+
+namespace cbc {
+
+const void* group_pick_n2(int nk, int field);
+const void* group_pick_n3(int nk, int field);
+const void* group_pick_n4(int nk, int field);
+const void* group_pick_n8(int nk, int field);
+const void* group_pick_n16(int nk, int field);
+
+static int pow2_up(int v)
+{
+    int t = 1;
+    while (t < v) t <<= 1;
+    return t;
+}
+
+static const void* group_pick_any(int ups, int nk, int field)
+{
+    switch (ups) {
+    case 2: return group_pick_n2(nk, field);
+    case 3: return group_pick_n3(nk, field);
+    case 4: return group_pick_n4(nk, field);
+    case 8: return group_pick_n8(nk, field);
+    case 16: return group_pick_n16(nk, field);
+    default: return nullptr;
+    }
+}
+
+bool group_supported(int ups, int nk, int field) { return group_pick_any(ups, nk, field) != nullptr; }
+
+int group_systems_per_block(int ups) { return 64 / pow2_up(ups); }
+
+hipError_t group_launch(const LaunchArgs& a, hipStream_t stream)
+{
+    const void* fn = group_pick_any(a.ups, a.nk, a.field);
+    if (!fn) return hipErrorInvalidValue;
+    const int spb = group_systems_per_block(a.ups);
+    const int grid = (a.ns + spb - 1) / spb;
+    void* args[] = {const_cast<LaunchArgs*>(&a)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, stream);
+}
+
+hipError_t group_attr(int ups, int nk, int field, KernelAttr* out)
+{
+    const void* fn = group_pick_any(ups, nk, field);
+    if (!fn) return hipErrorInvalidValue;
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    out->num_regs = fa.numRegs;
+    out->local_bytes = (int32_t)fa.localSizeBytes;
+    out->max_threads = fa.maxThreadsPerBlock;
+    out->lds_bytes = (int32_t)fa.sharedSizeBytes;
+    return hipSuccess;
+}
+
+}  // namespace cbc

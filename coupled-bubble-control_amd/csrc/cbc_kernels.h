@@ -55,4 +55,10 @@ hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream);
 hipError_t lane_attr(int ups, int nk, int field, KernelAttr* out);
 bool lane_supported(int ups, int nk, int field);
 
+// Group mapping: next_pow2(ups) lanes per system, one wave per workgroup.
+hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
+hipError_t group_attr(int ups, int nk, int field, KernelAttr* out);
+bool group_supported(int ups, int nk, int field);
+int group_systems_per_block(int ups);
+
 }  // namespace cbc

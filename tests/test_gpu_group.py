@@ -1,0 +1,94 @@
+"""GPU parity of the group mapping (one lane per bubble, LDS-staged Gauss--Jordan
+coupling solve) against the C oracle, for the bubble counts of the mixed and
+large-coupling workloads.  Same bars as test_gpu_plain.py."""
+import numpy as np
+import pytest
+
+from test_gpu_plain import RTOL, _assert_counts, _assert_states, _run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed(n, ns, periods, seed=2):
+    from coupledbubble_control import workloads
+
+    ws = {w.n: w for w in workloads.mixed_uneven(seed=seed, total=15 * 64)}
+    w = ws[n].subset(np.arange(ns))
+    w.t_end[:] = periods
+    return w
+
+
+@pytest.mark.parametrize("n,ns,periods", [(2, 128, 2.0), (3, 96, 2.0), (4, 64, 1.5), (8, 32, 1.0)])
+def test_group_mapping_parity(device, n, ns, periods):
+    from coupledbubble_control.backends import KernelVariant
+
+    print(f"case: group mapping N={n} {ns} systems {periods} periods", flush=True)
+    w = _mixed(n, ns, periods)
+    m, ref, cnt = _run_pair(w, variant=KernelVariant.GROUP)
+    info = m.runtime.kernel.info()
+    assert info.mapping == "group" and info.local_bytes == 0
+    assert np.array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+    _assert_states(m, ref, n)
+    _assert_counts(m, cnt)
+
+
+def test_group_n16_large_coupling(device):
+    """N = 16 (32 x 32 coupling solve per RHS evaluation)."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+
+    print("case: group mapping N=16, 12 systems, 0.25 periods", flush=True)
+    w16 = workloads.large_coupling(seed=4, ns16=12, ns2=4)[0]
+    w16.t_end[:] = 0.25
+    m, ref, cnt = _run_pair(w16, variant=KernelVariant.GROUP)
+    assert np.array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+    _assert_states(m, ref, 16)
+    _assert_counts(m, cnt)
+
+
+def test_group_and_lane_agree_for_n2(device):
+    """Both mappings of the same N=2 ensemble stay within the bar of each other."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+
+    print("case: lane vs group N=2, 128 systems, 2 periods", flush=True)
+    w = workloads.plain_cpu(seed=21, ns=128)
+    w.t_end[:] = 2.0
+    ml = workloads.build_model(w, variant=KernelVariant.LANE)
+    mg = workloads.build_model(w, variant=KernelVariant.GROUP)
+    for m in (ml, mg):
+        m.solve(max_steps=10 ** 7, kernel_steps=4096, sync_to_host=True)
+    a, b = ml.runtime.host_buffers, mg.runtime.host_buffers
+    np.testing.assert_array_equal(a.status.status_flags, b.status.status_flags)
+    want = a.state.actual_state[:, :, :2]
+    scale = np.maximum(np.abs(want).max(axis=2, keepdims=True), 1e-3 * np.abs(want).max(axis=(1, 2), keepdims=True))
+    assert np.all(np.abs(b.state.actual_state[:, :, :2] - want) <= RTOL * (scale + 1e-9))
+
+
+def test_group_dense_and_windows(device):
+    """Dense output and period windows through the group kernel."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+    from oracle import ckm1d_oracle as O
+
+    print("case: group N=4 windows, 32 systems, 3 periods", flush=True)
+    w = _mixed(4, 32, 3.0)
+    m = workloads.build_model(w, variant=KernelVariant.GROUP)
+    ref = O.host_buffers_dict(m)
+    out = O.solve_periodic(O.spec_from_model(m), ref, 1.0, 1, 2)
+    max_r, mean_x = m.solve_periodic(period=1.0, first_window=1, n_windows=2)
+    torch.cuda.synchronize()
+    flags = m.runtime.device_buffers.status.status_flags.cpu().numpy()
+    np.testing.assert_array_equal(flags, ref["status_flags"])
+    ok = flags == 1
+    mr = max_r.cpu().numpy()[:, ok, :4]
+    rel = np.abs(mr - out["max_r"][:, ok, :4]) / out["max_r"][:, ok, :4]
+    assert np.mean(rel <= 1e-6) >= 0.95 and rel.max() <= 2e-4
+    np.testing.assert_allclose(mean_x.cpu().numpy()[:, ok, :4], out["mean_x"][:, ok, :4], rtol=1e-6, atol=1e-12)
+
+    print("case: group N=3 dense output ndo=8", flush=True)
+    w3 = _mixed(3, 32, 1.0)
+    m3, ref3, _ = _run_pair(w3, ndo=8, variant=KernelVariant.GROUP)
+    np.testing.assert_array_equal(m3.runtime.host_buffers.dense.dense_index, ref3["dense_index"])

@@ -38,8 +38,11 @@ def _run_pair(w, kernel_steps=2048, max_steps=10 ** 8, ndo=0, variant=None):
 
 
 def _assert_states(m, ref, n, rtol=RTOL):
-    got = m.runtime.host_buffers.state.actual_state[:, :, :n]
-    want = ref["actual_state"][:, :, :n]
+    """States of systems the oracle finished normally (SUCCESS, EVENT_TERMINAL or still
+    ACTIVE); a STEP_SIZE_LIMIT or NaN stop leaves the state at a singular point."""
+    keep = ref["status_flags"] >= 0
+    got = m.runtime.host_buffers.state.actual_state[:, keep, :n]
+    want = ref["actual_state"][:, keep, :n]
     err = np.abs(got - want) / _scale(want)
     worst = np.unravel_index(np.argmax(err), err.shape)
     assert err.max() <= rtol, f"max rel err {err.max():.3e} at {worst}: got {got[worst]} want {want[worst]}"
