@@ -23,11 +23,15 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: no packed FP32 instructions are generated (a known-bad
 # instruction class on this platform); harmless for FP64 code.
+# -ffp-contract=on: a*b+c is fused into an FMA only within one source
+# expression (decided by the front end), never across statements after code
+# motion, so every inlined copy of the step code rounds identically.
 CFLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",
 ]

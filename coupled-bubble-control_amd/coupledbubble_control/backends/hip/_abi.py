@@ -97,6 +97,10 @@ SIGNATURES = {
     "cbc_solve": (_i32, [_vp, ctypes.POINTER(BuffersC), _i32, _i64, _vp,
                          ctypes.POINTER(_f64), _i32, ctypes.POINTER(_i32)]),
     "cbc_solve_periodic": (_i32, [_vp, ctypes.POINTER(BuffersC), ctypes.POINTER(PeriodSpecC), _i64, _vp]),
+    "cbc_solve_queued": (_i32, [_vp, ctypes.POINTER(BuffersC), _vp, _i32, _i32, _i32, _i32, _i64, _vp,
+                                ctypes.POINTER(_i32)]),
+    "cbc_launch_queued": (_i32, [_vp, ctypes.POINTER(BuffersC), _vp, _i32, _i32, _i32, _i32, _vp]),
+    "cbc_plan_active_count": (_i32, [_vp, _vp, ctypes.POINTER(_i32)]),
 }
 
 

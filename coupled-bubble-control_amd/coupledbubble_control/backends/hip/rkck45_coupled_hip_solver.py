@@ -136,3 +136,61 @@ def solve_periodic(kernel, d_buffers, period: float, first_window: int, max_r: t
                           _abi.ptr(mean_x) if nw else None)
     _abi.check(lib.cbc_solve_periodic(kernel.handle, ctypes.byref(bufs), ctypes.byref(ps), int(max_steps),
                                       _abi.stream_handle(stream)))
+
+
+def horizon_order(d_buffers) -> torch.Tensor:
+    """System indices by descending remaining horizon (time_end - actual_time), int32 on
+    the device: the longest-processing-time-first order for the work queue."""
+    rem = d_buffers.state.time_end - d_buffers.state.actual_time
+    return torch.argsort(rem, descending=True, stable=True).to(torch.int32)
+
+
+def _check_order(order: torch.Tensor, ns: int, validate: bool):
+    if not isinstance(order, torch.Tensor) or order.dtype != torch.int32 or not order.is_cuda:
+        raise TypeError("order must be an int32 tensor on the HIP device")
+    if order.dim() != 1 or order.numel() < 1 or order.numel() > ns or not order.is_contiguous():
+        raise ValueError(f"order must be a contiguous 1-D list of 1..{ns} system indices")
+    if validate:
+        if int(order.min().item()) < 0 or int(order.max().item()) >= ns:
+            raise ValueError("order holds indices outside [0, ns)")
+        if torch.unique(order).numel() != order.numel():
+            raise ValueError("order holds duplicate system indices (two lanes would race on one system)")
+
+
+def solve_queued(kernel, d_buffers, order: Optional[torch.Tensor] = None, waves: int = 0, refill: int = 1,
+                 kernel_steps: int = 1 << 20, max_steps: int = 1 << 24, stream=None, validate: bool = True) -> int:
+    """Persistent work-queue solve: lanes (or lane groups) take systems from ``order``
+    (default: longest remaining horizon first) and refill as systems stop.  Relaunches
+    while systems remain ACTIVE (at most ceil(max_steps/kernel_steps) launches).  Returns
+    the launch count."""
+    lib = _abi.lib()
+    _check_shapes(d_buffers, kernel.model_spec, kernel.execution_spec)
+    _check_trig_range(d_buffers, kernel.model_spec)
+    ns = kernel.execution_spec.ns
+    if order is None:
+        order = horizon_order(d_buffers)
+    _check_order(order, ns, validate)
+    bufs = _buffers_c(d_buffers)
+    n = ctypes.c_int32(0)
+    _abi.check(lib.cbc_solve_queued(kernel.handle, ctypes.byref(bufs), _abi.ptr(order), int(order.numel()),
+                                    int(waves), int(refill), int(kernel_steps), int(max_steps),
+                                    _abi.stream_handle(stream), ctypes.byref(n)))
+    return n.value
+
+
+def launch_queued(kernel, d_buffers, order: torch.Tensor, waves: int = 0, refill: int = 1,
+                  kernel_steps: int = 1 << 20, stream=None) -> None:
+    """One asynchronous persistent launch (no host synchronisation; capturable).  The
+    order tensor must stay alive until the launch has run."""
+    lib = _abi.lib()
+    bufs = _buffers_c(d_buffers)
+    _abi.check(lib.cbc_launch_queued(kernel.handle, ctypes.byref(bufs), _abi.ptr(order), int(order.numel()),
+                                     int(waves), int(refill), int(kernel_steps), _abi.stream_handle(stream)))
+
+
+def active_count(kernel, stream=None) -> int:
+    """Systems left ACTIVE by the plan's last launch (synchronises the stream)."""
+    lib = _abi.lib()
+    out = ctypes.c_int32(0)
+    _abi.check(lib.cbc_plan_active_count(kernel.handle, _abi.stream_handle(stream), ctypes.byref(out)))
+    return out.value

@@ -1,0 +1,181 @@
+"""Ensembles of systems with different bubble counts (the batched uneven workload).
+
+The reference compiles one kernel per bubble count (UPS is a compile-time macro,
+backends/cupy_cuda/kernel_loader.py:49) and can integrate only one bubble count
+per model.  Here an ensemble holds one CKM1D model per bubble count ("bin", each
+in the reference buffer layout) and integrates all of them together:
+
+* each bin runs one persistent work-queue launch (cbc_launch_queued): lanes or
+  lane groups take that bin's systems longest-horizon-first and refill as
+  systems stop, so uneven horizons and step counts do not idle lanes;
+* the bins' launches are issued on separate streams without host
+  synchronisation between them, so when one bin's queue drains its waves free
+  SIMDs for the others (the tails overlap);
+* the launch sequence can be captured once into a HIP graph and replayed;
+* ``autotune`` picks, per bin, the mapping (lane vs group), the refill
+  threshold and the persistent grid from short timed trials.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from dataclasses import asdict, dataclass, field
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import torch
+
+from .backends import KernelVariant
+from .backends.hip import rkck45_coupled_hip_solver as hs
+
+
+@dataclass
+class BinTuning:
+    """Launch choice for one bin."""
+    variant: str = "auto"      # "lane" | "group" | "auto"
+    refill: int = 1            # free lanes (groups) per wave that trigger a refill
+    waves: int = 0             # persistent grid in waves (0 = fill every SIMD)
+
+
+@dataclass
+class TuneRecord:
+    n: int
+    choice: BinTuning
+    trials: Dict[str, float] = field(default_factory=dict)
+
+
+class MixedEnsemble:
+    def __init__(self, models: Sequence, tuning: Optional[Dict[int, BinTuning]] = None):
+        self.models = list(models)
+        ns = [m._execution_spec.ups for m in self.models]
+        if len(set(ns)) != len(ns):
+            raise ValueError("one model per bubble count")
+        self.tuning = {m._execution_spec.ups: (tuning or {}).get(m._execution_spec.ups, BinTuning())
+                       for m in self.models}
+        self._streams: List[torch.cuda.Stream] = []
+        self._orders: List[torch.Tensor] = []
+        self._graph = None
+
+    # -- set-up ---------------------------------------------------------------
+    def _prepare(self):
+        for m in self.models:
+            m._ensure_buffers()
+            want = KernelVariant(self.tuning[m._execution_spec.ups].variant)
+            if m.runtime.kernel is None or m.runtime.variant != want:
+                m.runtime.variant = want
+                m.runtime.solver = m._make_solver(m.runtime.backend, want)
+        if len(self._streams) != len(self.models):
+            self._streams = [torch.cuda.Stream() for _ in self.models]
+
+    def orders(self) -> List[torch.Tensor]:
+        """Longest-remaining-horizon-first queues of every bin (kept alive for replays)."""
+        self._orders = [hs.horizon_order(m.runtime.device_buffers) for m in self.models]
+        return self._orders
+
+    # -- execution ------------------------------------------------------------
+    def launch(self, kernel_steps: int = 1 << 20, orders: Optional[List[torch.Tensor]] = None) -> None:
+        """Issue one persistent launch per bin on its own stream, joined back into the
+        current stream (asynchronous; capturable)."""
+        self._prepare()
+        orders = orders if orders is not None else self.orders()
+        main = torch.cuda.current_stream()
+        for m, st, order in zip(self.models, self._streams, orders):
+            t = self.tuning[m._execution_spec.ups]
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                hs.launch_queued(m.runtime.kernel, m.runtime.device_buffers, order, waves=t.waves,
+                                 refill=t.refill, kernel_steps=kernel_steps, stream=st)
+        for st in self._streams:
+            main.wait_stream(st)
+
+    def solve(self, kernel_steps: int = 1 << 20, max_launches: int = 64) -> int:
+        """Launch until no system of any bin is ACTIVE; returns the launch rounds."""
+        rounds = 0
+        for _ in range(max_launches):
+            self.launch(kernel_steps)
+            rounds += 1
+            torch.cuda.current_stream().synchronize()
+            left = sum(hs.active_count(m.runtime.kernel, st) for m, st in zip(self.models, self._streams))
+            if left == 0:
+                break
+        return rounds
+
+    def capture(self, kernel_steps: int = 1 << 20):
+        """Capture one launch round into a HIP graph (replayed by ``replay``).  The
+        queue order is fixed at capture time."""
+        self._prepare()
+        orders = self.orders()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch(kernel_steps, orders=orders)
+        self._graph = g
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        self._graph.replay()
+
+    # -- results --------------------------------------------------------------
+    def integrated_periods(self) -> float:
+        return float(sum((m.runtime.device_buffers.state.actual_time.sum()).item() for m in self.models))
+
+    def attempts(self) -> Dict[int, int]:
+        return {m._execution_spec.ups: int(m.runtime.device_buffers.status.total_steps.to(torch.int64).sum().item())
+                for m in self.models}
+
+
+def _trial_copy(m, horizon: float, limit: int):
+    """A bin copy of at most `limit` systems integrating `horizon` periods."""
+    import numpy as np
+
+    from .models.ckm1d import CKM1D
+
+    es = m._execution_spec
+    ns = min(es.ns, limit)
+    hb = m.runtime.host_buffers
+    R0 = m._R0[:ns, :es.ups] * 1e6
+    c = CKM1D(num_systems=ns, systems_per_block=64, units_per_system=es.ups, R0=R0,
+              PA=m._PA[:, :ns] / 1e5, FR=m._FREQ[:, :ns] / 1e3, PS=m._PS[:, :ns] / np.pi,
+              REF_FREQ=m._REF_FREQ / 1e3, num_components=m._model_spec.nk, acoustic_field=m._model_spec.ac,
+              mat_props=m.mat_props, solver_spec=m._solver_spec)
+    c.runtime.host_buffers.state.actual_state[:] = hb.state.actual_state[:, :ns]
+    c.runtime.host_buffers.state.time_end[:] = np.minimum(hb.state.time_end[:ns], horizon)
+    return c
+
+
+def autotune(models: Sequence, horizon: float = 0.5, limit: int = 8192,
+             refills: Sequence[int] = (1, 16, 64), cache: Optional[Path] = None) -> Dict[int, TuneRecord]:
+    """Pick mapping and refill threshold per bin from timed trials of `horizon` periods on
+    up to `limit` systems of that bin (the group mapping is the only choice for N >= 3
+    in this build; the lane mapping spills beyond N = 2).  Results are cached as JSON
+    keyed by bubble count when `cache` is given."""
+    if cache is not None and Path(cache).exists():
+        data = json.loads(Path(cache).read_text())
+        return {int(k): TuneRecord(v["n"], BinTuning(**v["choice"]), v["trials"]) for k, v in data.items()}
+    out: Dict[int, TuneRecord] = {}
+    for m in models:
+        n = m._execution_spec.ups
+        variants = ["lane", "group"] if n <= 2 else ["group"]
+        best, best_t, trials = None, float("inf"), {}
+        for var in variants:
+            for rf in refills:
+                c = _trial_copy(m, horizon, limit)
+                ens = MixedEnsemble([c], {n: BinTuning(var, rf, 0)})
+                ens._prepare()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ens.solve()
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                trials[f"{var}/refill={rf}"] = dt
+                if dt < best_t:
+                    best, best_t = BinTuning(var, rf, 0), dt
+        out[n] = TuneRecord(n, best, trials)
+    if cache is not None:
+        Path(cache).parent.mkdir(parents=True, exist_ok=True)
+        Path(cache).write_text(json.dumps({str(k): {"n": v.n, "choice": asdict(v.choice), "trials": v.trials}
+                                           for k, v in out.items()}, indent=1))
+    return out

@@ -43,6 +43,20 @@ int next_pow2(int v)
     return t;
 }
 
+// Clears n 32-bit words; a kernel rather than hipMemsetAsync so that launch
+// sequences stay valid under stream capture.
+__global__ void zero_words(uint32_t* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t zero_async(uint32_t* p, int n, hipStream_t st)
+{
+    void* args[] = {&p, &n};
+    return hipLaunchKernel(reinterpret_cast<const void*>(&zero_words), dim3((n + 63) / 64), dim3(64), args, 0, st);
+}
+
 }  // namespace
 
 struct cbc_plan {
@@ -211,6 +225,10 @@ static int32_t fill_args(const cbc_plan* p, const cbc_buffers* b, cbc::LaunchArg
     a.total_steps = b->total_steps;
     a.rejected_steps = b->rejected_steps;
     a.period = 0.0;
+    a.queue = nullptr;
+    a.queue_len = 0;
+    a.refill = 1;
+    a.queue_head = nullptr;
     return CBC_OK;
 }
 
@@ -244,8 +262,8 @@ int32_t cbc_solve(cbc_plan* plan, const cbc_buffers* buf, int32_t kernel_steps, 
     int32_t launches = 0;
     for (int64_t l = 0; l < max_launches; ++l) {
         auto t0 = std::chrono::steady_clock::now();
-        hipError_t e = hipMemsetAsync(plan->d_active, 0, sizeof(uint32_t), st);
-        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(active counter)");
+        hipError_t e = zero_async(plan->d_active, 1, st);
+        if (e != hipSuccess) return hip_fail(e, "zero active counter");
         e = launch_kernel(plan, a, st);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         e = hipMemcpyAsync(plan->h_active, plan->d_active, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
@@ -284,6 +302,96 @@ int32_t cbc_solve_periodic(cbc_plan* plan, const cbc_buffers* buf, const cbc_per
     a.active_count = nullptr;
     hipError_t e = launch_kernel(plan, a, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch (periodic)");
+    return CBC_OK;
+}
+
+static int32_t queued_args(cbc_plan* plan, const cbc_buffers* buf, const int32_t* order, int32_t order_len,
+                           int32_t& waves, int32_t refill, int32_t kernel_steps, cbc::LaunchArgs& a)
+{
+    if (!plan) return fail(CBC_ERR_INVALID, "queued solve: null plan");
+    if (!order || order_len < 1) return fail(CBC_ERR_INVALID, "order must hold at least one system index");
+    if (order_len > plan->exec.ns) return fail(CBC_ERR_INVALID, "order_len %d exceeds ns %d", order_len, plan->exec.ns);
+    if (kernel_steps < 1) return fail(CBC_ERR_INVALID, "kernel_steps must be >= 1, got %d", kernel_steps);
+    if (refill < 1 || refill > 64) return fail(CBC_ERR_INVALID, "refill must be in [1, 64], got %d", refill);
+    if (waves < 0) return fail(CBC_ERR_INVALID, "waves must be >= 0");
+    int32_t rc = fill_args(plan, buf, a);
+    if (rc != CBC_OK) return rc;
+    const bool lane = plan->mapping == CBC_MAP_LANE;
+    if (waves == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_fail(e, "device query");
+        cbc::KernelAttr at{};
+        e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
+        // waves resident per SIMD from the register budget (512 per lane on gfx950)
+        const int regs = at.num_regs > 0 ? at.num_regs : 512;
+        per_cu = 4 * (512 / regs > 0 ? 512 / regs : 1);
+        waves = cus * per_cu;
+    }
+    a.kernel_steps = kernel_steps;
+    a.queue = order;
+    a.queue_len = order_len;
+    a.refill = refill;
+    a.queue_head = plan->d_active + 1;
+    a.active_count = plan->d_active;
+    return CBC_OK;
+}
+
+static hipError_t queued_launch(cbc_plan* plan, const cbc::LaunchArgs& a, int32_t waves, hipStream_t st)
+{
+    hipError_t e = zero_async(plan->d_active, 2, st);
+    if (e != hipSuccess) return e;
+    return plan->mapping == CBC_MAP_LANE ? cbc::lane_launch_persistent(a, plan->block, waves, st)
+                                         : cbc::group_launch_persistent(a, waves, st);
+}
+
+int32_t cbc_launch_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t* order, int32_t order_len,
+                          int32_t waves, int32_t refill, int32_t kernel_steps, void* stream)
+{
+    cbc::LaunchArgs a;
+    int32_t rc = queued_args(plan, buf, order, order_len, waves, refill, kernel_steps, a);
+    if (rc != CBC_OK) return rc;
+    hipError_t e = queued_launch(plan, a, waves, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "persistent kernel launch");
+    return CBC_OK;
+}
+
+int32_t cbc_plan_active_count(cbc_plan* plan, void* stream, int32_t* out)
+{
+    if (!plan || !out) return fail(CBC_ERR_INVALID, "cbc_plan_active_count: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(plan->h_active, plan->d_active, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "active count readback");
+    *out = (int32_t)*plan->h_active;
+    return CBC_OK;
+}
+
+int32_t cbc_solve_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t* order, int32_t order_len,
+                         int32_t waves, int32_t refill, int32_t kernel_steps, int64_t max_steps, void* stream,
+                         int32_t* n_launches)
+{
+    if (max_steps < 1) return fail(CBC_ERR_INVALID, "max_steps must be >= 1");
+    cbc::LaunchArgs a;
+    int32_t rc = queued_args(plan, buf, order, order_len, waves, refill, kernel_steps, a);
+    if (rc != CBC_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t max_launches = (max_steps + kernel_steps - 1) / kernel_steps;
+    int32_t launches = 0;
+    for (int64_t l = 0; l < max_launches; ++l) {
+        hipError_t e = queued_launch(plan, a, waves, st);
+        if (e != hipSuccess) return hip_fail(e, "persistent kernel launch");
+        e = hipMemcpyAsync(plan->h_active, plan->d_active, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(active counter)");
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        ++launches;
+        if (*plan->h_active == 0) break;
+    }
+    if (n_launches) *n_launches = launches;
     return CBC_OK;
 }
 

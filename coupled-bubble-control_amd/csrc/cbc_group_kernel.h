@@ -4,14 +4,305 @@
 //
 // Step logic is that of the reference stepper (rkck45_coupled_numba_stepper.py:179-706)
 // as in cbc_lane.hip; every per-system quantity (t, h, event value, error ratio,
-// flags) is held redundantly by all lanes of the group.  The whole wave keeps
-// looping until every system in it has stopped, so barriers stay uniform;
-// lanes of stopped systems compute but never store.
+// flags) is held redundantly by all lanes of the group.  The whole wave runs
+// every attempt together (the RHS has barriers), so lanes of idle groups
+// compute but never store.
+//
+// Plain mode: group = system blockIdx.x * S + group index.
+// Persistent mode: free groups take systems from the work queue (one atomic per
+// refill by the wave, consecutive slots to free groups in lane order) once at
+// least `refill` groups are free, or all are.
 #pragma once
 
 #include "cbc_group.h"
 
 namespace cbc {
+
+template <int N, int NK>
+struct GroupRun {
+    GroupParams<NK> P;
+    double y[4];
+    double t, t_end, h, g_old;
+    int s, flag, attempts, rejected;
+    bool vu;  // lane carries a real bubble of a loaded system
+    int didx;
+    double dnext, ddt;
+    int wcur;
+    double wend, wmax, wsum;
+};
+
+// Per-lane load of system s into group slot L (no barrier).  Returns whether the
+// system is active; a system finished at load gets its flag written.
+template <int N, int NK>
+__device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, GroupLds<N>& L,
+                                            GroupRun<N, NK>& R)
+{
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    const size_t NST = NS * TILE;
+    const size_t base = (size_t)s * TILE;
+    R.s = s;
+    R.vu = i < N;
+    const int iu = R.vu ? i : 0;
+    R.t = a.t[s];
+    R.t_end = a.t_end[s];
+    R.h = a.h[s];
+    R.g_old = a.event[s];
+    R.flag = kActive;
+    R.attempts = 0;
+    R.rejected = 0;
+    bool active = true;
+    if (R.t >= R.t_end) {
+        R.flag = kSuccess;
+        active = false;
+    }
+    if (fabs(R.g_old) < a.etol) {
+        R.flag = kEventTerminal;
+        active = false;
+    }
+    if (!active) {
+        if (i == 0) a.flags[s] = R.flag;
+        R.vu = false;
+        return false;
+    }
+    GroupParams<NK>& P = R.P;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) P.up.u[j] = a.up[j * NST + base + iu];
+    P.rcol = a.up[12 * NST + base + iu];
+    P.sp0 = a.sp[s];
+    P.sp3 = a.sp[3 * NS + s];
+    {
+        const double TWO_PI = 6.28318530717958647692;
+        const double w0 = TWO_PI * a.sp[NS + s];
+        const double l0 = TWO_PI * a.sp[2 * NS + s];
+        const double sp4 = a.sp[4 * NS + s];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const double amps = a.cp[(size_t)k * NS + s];
+            const double omega = a.cp[(size_t)(NK + k) * NS + s];
+            const double phase = a.cp[(size_t)(2 * NK + k) * NS + s];
+            const double wave = a.cp[(size_t)(3 * NK + k) * NS + s];
+            P.d.wt[k] = w0 * omega;
+            P.d.wx[k] = l0 * wave;
+            P.d.ph[k] = phase;
+            P.d.A[k] = amps;
+            P.d.Aw[k] = amps * omega;
+            P.d.Ak[k] = amps * wave;
+            P.d.Au[k] = amps * sp4;
+        }
+    }
+    // coupling constants: coupling[c][j][s][i] is the effect of unit j on unit i
+    if (R.vu) {
+        const size_t cstride = (size_t)a.ups * NST;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const size_t off = ((size_t)j * NS + s) * TILE + i;
+            L.mx[0][i][j] = a.cm[off];
+            L.mx[1][i][j] = a.cm[cstride + off];
+            L.mx[2][i][j] = a.cm[2 * cstride + off];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        R.y[c] = R.vu ? a.state[c * NST + base + i] : (c == 0 ? 1.0 : (c == 1 ? 1e6 * (i + 1) : 0.0));
+
+    // dense output (rkck45_coupled_numba_stepper.py:72-112, 333-345)
+    R.didx = 0;
+    R.dnext = R.t;
+    R.ddt = 0.0;
+    if (a.ndo > 0) {
+        const double tb = a.t_begin[s];
+        R.didx = a.dense_index[s];
+        R.ddt = (a.ndo == 1) ? (R.t_end - tb) : (R.t_end - tb) / (a.ndo - 1);
+        if (R.didx < a.ndo && R.dnext <= R.t) {
+            if (i == 0) a.dense_time[(size_t)R.didx * NS + s] = R.t;
+            if (R.vu)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    a.dense_state[(((size_t)R.didx * 4 + c) * NS + s) * TILE + i] = R.y[c];
+            ++R.didx;
+            R.dnext = fmin(R.t + R.ddt, R.t_end);
+        }
+    }
+    // period windows (resumable; see cbc_lane.hip)
+    R.wcur = 0;
+    R.wend = R.t_end;
+    R.wmax = R.y[0];
+    R.wsum = 0.0;
+    if (a.period > 0.0) {
+        R.wcur = (int)floor(R.t / a.period + 1e-12);
+        R.wend = fmin((R.wcur + 1) * a.period, R.t_end);
+        const int widx = R.wcur - a.first_window;
+        if (R.vu && (R.t - R.wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
+            R.wmax = a.max_r[((size_t)widx * NS + s) * TILE + i];
+            R.wsum = a.mean_x[((size_t)widx * NS + s) * TILE + i] * a.period;
+        }
+    }
+    return true;
+}
+
+// One attempt for every group of the wave (uniform: contains barriers).  `busy`
+// marks groups with a live system; returns true for a busy group whose system
+// stopped this attempt.
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupLds<N>& L, GroupRun<N, NK>& R,
+                                              bool busy)
+{
+    constexpr int G = GroupShape<N>::G;
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    double& t = R.t;
+    double& h = R.h;
+    double (&y)[4] = R.y;
+    const bool vu = busy && R.vu;
+    if (busy) ++R.attempts;
+    if (h < a.min_step) h = a.min_step;
+    if (h > a.max_step) h = a.max_step;
+    {
+        const double remain = R.wend - t;
+        if (h > remain) h = remain;
+    }
+
+    double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], tmp[4];
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t, y, k1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a2 * h, tmp, k2);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a3 * h, tmp, k3);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a4 * h, tmp, k4);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a5 * h, tmp, k5);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
+                             CK::b65 * k5[c]);
+    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a6 * h, tmp, k6);
+
+    double ynew[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
+                              CK::c5 * k5[c] + CK::c6 * k6[c]);
+    const double g_new = group_event<N>(L, i, vu, R.P.rcol, ynew, a.col_factor);
+
+    // error ratio of the lane's bubble, reduced over the group
+    double ratio_u = 1.0e30;
+    bool bad_u = false;
+    if (vu) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
+                                       CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
+            const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
+            const double r = tol * rcp_fast(e);
+            bad_u |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);
+            if (r < ratio_u) ratio_u = r;
+        }
+    }
+    const double ratio = group_min<G>(ratio_u);
+    const bool bad = group_any<G>(bad_u);
+    if (!busy) return false;
+
+    const int s = R.s;
+    const bool crossing = R.g_old * g_new < 0.0;
+    if (crossing) {
+        if (h <= a.min_step) {
+            R.flag = kStepSizeLimit;
+            return true;
+        }
+        const double aold = fabs(R.g_old);
+        h = h * ((aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99);
+        ++R.rejected;
+    } else {
+        if (bad) {
+            R.flag = kNaN;
+            return true;
+        }
+        const bool ok = ratio >= 1.0;
+        if (ok ? (t + h == t) : (h <= a.min_step)) {
+            R.flag = kStepSizeLimit;
+            return true;
+        }
+        const double fac = step_factor(ok, ratio);
+        if (ok) {
+            if (a.period > 0.0) {
+                if (ynew[0] > R.wmax) R.wmax = ynew[0];
+                R.wsum += 0.5 * h * (y[1] + ynew[1]);
+            }
+            t = t + h;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) y[c] = ynew[c];
+            R.g_old = g_new;
+        } else {
+            ++R.rejected;
+        }
+        h = h * fac;
+    }
+    if (a.ndo > 0 && R.didx < a.ndo && R.dnext <= t) {
+        if (i == 0) a.dense_time[(size_t)R.didx * NS + s] = t;
+        if (R.vu)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a.dense_state[(((size_t)R.didx * 4 + c) * NS + s) * TILE + i] = y[c];
+        ++R.didx;
+        R.dnext = fmin(t + R.ddt, R.t_end);
+    }
+    if (a.period > 0.0 && t >= R.wend) {
+        const int widx = R.wcur - a.first_window;
+        if (R.vu && widx >= 0 && widx < a.n_windows) {
+            a.max_r[((size_t)widx * NS + s) * TILE + i] = R.wmax;
+            a.mean_x[((size_t)widx * NS + s) * TILE + i] = R.wsum / a.period;
+        }
+        ++R.wcur;
+        R.wend = fmin((R.wcur + 1) * a.period, R.t_end);
+        R.wmax = y[0];
+        R.wsum = 0.0;
+    }
+    if (t >= R.t_end) {
+        R.flag = kSuccess;
+        return true;
+    }
+    if (fabs(R.g_old) < a.etol) {
+        R.flag = kEventTerminal;
+        return true;
+    }
+    return false;
+}
+
+template <int N, int NK>
+__device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const GroupRun<N, NK>& R)
+{
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    const size_t NST = NS * TILE;
+    const int s = R.s;
+    const size_t base = (size_t)s * TILE;
+    if (a.period > 0.0 && R.flag != kSuccess) {
+        const int widx = R.wcur - a.first_window;
+        if (R.vu && (R.t - R.wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
+            a.max_r[((size_t)widx * NS + s) * TILE + i] = R.wmax;
+            a.mean_x[((size_t)widx * NS + s) * TILE + i] = R.wsum / a.period;
+        }
+    }
+    if (R.vu)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a.state[c * NST + base + i] = R.y[c];
+    if (i == 0) {
+        a.t[s] = R.t;
+        a.h[s] = R.h;
+        a.event[s] = R.g_old;
+        a.flags[s] = R.flag;
+        if (a.ndo > 0) a.dense_index[s] = R.didx;
+        if (a.total_steps) a.total_steps[s] += R.attempts;
+        if (a.rejected_steps) a.rejected_steps[s] += R.rejected;
+        if (R.flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+    }
+}
 
 template <int N, int NK, int FIELD>
 __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
@@ -24,267 +315,78 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
     const int lane = threadIdx.x;
     const int ls = lane / G;
     const int i = lane % G;
-    const int s = blockIdx.x * S + ls;
-    const bool vsys = s < a.ns;
-    const bool vu = vsys && i < N;
     GroupLds<N>& L = lds[ls];
-    const size_t NS = (size_t)a.ns;
-    const size_t TILE = (size_t)a.tile;
-    const size_t NST = NS * TILE;
-    const size_t sc = vsys ? (size_t)s : 0;  // clamped index for safe loads
-    const size_t base = sc * TILE;
-    const int iu = vu ? i : 0;
-
-    // load and early termination (rkck45_coupled_numba_stepper.py:179-213)
-    double t = a.t[sc];
-    const double t_end = a.t_end[sc];
-    double h = a.h[sc];
-    double g_old = a.event[sc];
-    int flag = kActive;
-    bool active = vsys;
-    if (vsys) {
-        if (t >= t_end) {
-            flag = kSuccess;
-            active = false;
-        }
-        if (fabs(g_old) < a.etol) {
-            flag = kEventTerminal;
-            active = false;
-        }
-        if (!active && i == 0) a.flags[s] = flag;
-    }
-    if (!__any(active)) return;  // wave-uniform
-
-    GroupParams<NK> P;
+    GroupRun<N, NK> R;
+    R.vu = false;
+    R.y[0] = 1.0;
+    R.y[1] = 1e6 * (i + 1);
+    R.y[2] = 0.0;
+    R.y[3] = 0.0;
+    R.t = 0.0;
+    R.h = 1e-3;
+    R.wend = 1.0;
+    R.P.rcol = 0.0;
+    R.P.sp0 = 4.2;
+    R.P.sp3 = 1.0;
 #pragma unroll
-    for (int j = 0; j < 12; ++j) P.up.u[j] = a.up[j * NST + base + iu];
-    P.rcol = a.up[12 * NST + base + iu];
-    P.sp0 = a.sp[sc];
-    P.sp3 = a.sp[3 * NS + sc];
-    {
-        const double TWO_PI = 6.28318530717958647692;
-        const double w0 = TWO_PI * a.sp[NS + sc];
-        const double l0 = TWO_PI * a.sp[2 * NS + sc];
-        const double sp4 = a.sp[4 * NS + sc];
+    for (int j = 0; j < 12; ++j) R.P.up.u[j] = 1.0;
 #pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            const double amps = a.cp[(size_t)k * NS + sc];
-            const double omega = a.cp[(size_t)(NK + k) * NS + sc];
-            const double phase = a.cp[(size_t)(2 * NK + k) * NS + sc];
-            const double wave = a.cp[(size_t)(3 * NK + k) * NS + sc];
-            P.d.wt[k] = w0 * omega;
-            P.d.wx[k] = l0 * wave;
-            P.d.ph[k] = phase;
-            P.d.A[k] = amps;
-            P.d.Aw[k] = amps * omega;
-            P.d.Ak[k] = amps * wave;
-            P.d.Au[k] = amps * sp4;
-        }
-    }
-    // coupling constants: coupling[c][j][s][i] is the effect of unit j on unit i
-    if (vu) {
-        const size_t cstride = (size_t)a.ups * NST;
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const size_t off = ((size_t)j * NS + sc) * TILE + i;
-            L.mx[0][i][j] = a.cm[off];
-            L.mx[1][i][j] = a.cm[cstride + off];
-            L.mx[2][i][j] = a.cm[2 * cstride + off];
-        }
+    for (int k = 0; k < NK; ++k) {
+        R.P.d.wt[k] = R.P.d.wx[k] = R.P.d.ph[k] = 0.0;
+        R.P.d.A[k] = R.P.d.Aw[k] = R.P.d.Ak[k] = R.P.d.Au[k] = 0.0;
     }
 
-    double y[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) y[c] = vu ? a.state[c * NST + base + i] : (c == 0 ? 1.0 : (c == 1 ? 1e6 * (i + 1) : 0.0));
-    __syncthreads();
-
-    {
-        const double g0 = group_event<N>(L, i, vu, P.rcol, y, a.col_factor);
-        if (active) g_old = g0;
+    bool busy = false;
+    int64_t used = 0;
+    const bool persistent = a.queue != nullptr;
+    bool drained = !persistent;
+    if (!persistent) {
+        const int s = blockIdx.x * S + ls;
+        if (s < a.ns) busy = group_begin<N, NK>(a, s, i, L, R);
+        __syncthreads();
+        const double g0 = group_event<N>(L, i, busy && R.vu, R.P.rcol, R.y, a.col_factor);
+        if (busy) R.g_old = g0;
     }
-
-    // dense output (rkck45_coupled_numba_stepper.py:72-112, 333-345)
-    const bool dense = a.ndo > 0;
-    int didx = 0;
-    double dnext = t, ddt = 0.0;
-    if (dense && vsys) {
-        const double tb = a.t_begin[sc];
-        didx = a.dense_index[sc];
-        ddt = (a.ndo == 1) ? (t_end - tb) : (t_end - tb) / (a.ndo - 1);
-        if (active && didx < a.ndo && dnext <= t) {
-            if (i == 0) a.dense_time[(size_t)didx * NS + s] = t;
-            if (vu)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) a.dense_state[(((size_t)didx * 4 + c) * NS + s) * TILE + i] = y[c];
-            ++didx;
-            dnext = fmin(t + ddt, t_end);
-        }
-    }
-
-    // period windows (resumable; see cbc_lane.hip)
-    const bool win = a.period > 0.0;
-    int wcur = 0;
-    double wend = t_end;
-    double wmax = y[0], wsum = 0.0;
-    if (win && vsys) {
-        wcur = (int)floor(t / a.period + 1e-12);
-        wend = fmin((wcur + 1) * a.period, t_end);
-        const int widx = wcur - a.first_window;
-        if (vu && (t - wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
-            wmax = a.max_r[((size_t)widx * NS + s) * TILE + i];
-            wsum = a.mean_x[((size_t)widx * NS + s) * TILE + i] * a.period;
-        }
-    }
-
-    int attempts = 0, rejected = 0;
-    for (int64_t step = 0; step < a.kernel_steps; ++step) {
-        if (!__any(active)) break;
-        if (active) ++attempts;
-        if (h < a.min_step) h = a.min_step;
-        if (h > a.max_step) h = a.max_step;
-        {
-            const double remain = wend - t;
-            if (h > remain) h = remain;
-        }
-
-        double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], tmp[4];
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t, y, k1);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t + CK::a2 * h, tmp, k2);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t + CK::a3 * h, tmp, k3);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t + CK::a4 * h, tmp, k4);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t + CK::a5 * h, tmp, k5);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
-                                 CK::b65 * k5[c]);
-        group_rhs<N, NK, FIELD>(P, L, i, vu, t + CK::a6 * h, tmp, k6);
-
-        double ynew[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
-                                  CK::c5 * k5[c] + CK::c6 * k6[c]);
-        const double g_new = group_event<N>(L, i, vu, P.rcol, ynew, a.col_factor);
-
-        // error ratio of the lane's bubble, reduced over the group
-        double ratio_u = 1.0e30;
-        bool bad_u = false;
-        if (vu) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
-                                           CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
-                const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
-                const double r = tol * rcp_fast(e);
-                bad_u |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);
-                if (r < ratio_u) ratio_u = r;
-            }
-        }
-        const double ratio = group_min<G>(ratio_u);
-        const bool bad = group_any<G>(bad_u);
-
-        if (active) {
-            const bool crossing = g_old * g_new < 0.0;
-            bool stop = false;
-            if (crossing) {
-                if (h <= a.min_step) {
-                    flag = kStepSizeLimit;
-                    stop = true;
-                } else {
-                    const double aold = fabs(g_old);
-                    h = h * ((aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99);
-                    ++rejected;
+    while (true) {
+        if (!drained) {
+            const bool free_leader = !busy && i == 0;
+            const uint64_t freemask = __ballot(free_leader);
+            const int nfree = __popcll(freemask);
+            const bool any_busy = __any(busy);
+            if (nfree > 0 && (nfree >= a.refill || !any_busy)) {
+                const int leader = __ffsll((unsigned long long)freemask) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
+                base = __shfl(base, leader);
+                if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
+                const uint32_t rank = (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
+                uint32_t idx = free_leader ? base + rank : 0u;
+                idx = __shfl(idx, ls * G);  // group leader's slot to the whole group
+                const int q = (!busy && idx < (uint32_t)a.queue_len) ? a.queue[idx] : -1;
+                const bool gets = q >= 0 && q < a.ns;  // malformed entries are skipped
+                bool fresh = false;
+                if (gets) {
+                    fresh = group_begin<N, NK>(a, q, i, L, R);
+                    used = 0;
                 }
-            } else if (bad) {
-                flag = kNaN;
-                stop = true;
-            } else {
-                const bool ok = ratio >= 1.0;
-                if (ok ? (t + h == t) : (h <= a.min_step)) {
-                    flag = kStepSizeLimit;
-                    stop = true;
-                } else {
-                    const double fac = step_factor(ok, ratio);
-                    if (ok) {
-                        if (win) {
-                            if (ynew[0] > wmax) wmax = ynew[0];
-                            wsum += 0.5 * h * (y[1] + ynew[1]);
-                        }
-                        t = t + h;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) y[c] = ynew[c];
-                        g_old = g_new;
-                    } else {
-                        ++rejected;
-                    }
-                    h = h * fac;
-                }
-            }
-            if (stop) {
-                active = false;
-            } else {
-                if (dense && didx < a.ndo && dnext <= t) {
-                    if (i == 0) a.dense_time[(size_t)didx * NS + s] = t;
-                    if (vu)
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            a.dense_state[(((size_t)didx * 4 + c) * NS + s) * TILE + i] = y[c];
-                    ++didx;
-                    dnext = fmin(t + ddt, t_end);
-                }
-                if (win && t >= wend) {
-                    const int widx = wcur - a.first_window;
-                    if (vu && widx >= 0 && widx < a.n_windows) {
-                        a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax;
-                        a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum / a.period;
-                    }
-                    ++wcur;
-                    wend = fmin((wcur + 1) * a.period, t_end);
-                    wmax = y[0];
-                    wsum = 0.0;
-                }
-                if (t >= t_end) {
-                    flag = kSuccess;
-                    active = false;
-                } else if (fabs(g_old) < a.etol) {
-                    flag = kEventTerminal;
-                    active = false;
+                __syncthreads();
+                const double g0 = group_event<N>(L, i, fresh && R.vu, R.P.rcol, R.y, a.col_factor);
+                if (fresh) {
+                    R.g_old = g0;
+                    busy = true;
                 }
             }
         }
-    }
-
-    if (!vsys) return;
-    // systems that were stopped at load have nothing to store
-    if (attempts == 0 && flag != kActive) return;
-    if (win && flag != kSuccess) {
-        const int widx = wcur - a.first_window;
-        if (vu && (t - wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
-            a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax;
-            a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum / a.period;
+        if (!__any(busy)) {
+            if (drained) break;
+            continue;
         }
-    }
-    if (vu)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) a.state[c * NST + base + i] = y[c];
-    if (i == 0) {
-        a.t[s] = t;
-        a.h[s] = h;
-        a.event[s] = g_old;
-        a.flags[s] = flag;
-        if (dense) a.dense_index[s] = didx;
-        if (a.total_steps) a.total_steps[s] += attempts;
-        if (a.rejected_steps) a.rejected_steps[s] += rejected;
-        if (flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+        const bool stop = group_attempt<N, NK, FIELD>(a, i, L, R, busy);
+        if (busy && (stop || ++used >= a.kernel_steps)) {
+            group_finish<N, NK>(a, i, R);
+            busy = false;
+            R.vu = false;
+        }
     }
 }
 

@@ -40,6 +40,15 @@ struct LaunchArgs {
     double* mean_x;
 
     uint32_t* active_count;    // +1 per system left ACTIVE (may be null)
+
+    // persistent mode (queue != null): lanes / lane groups take system indices
+    // from queue[0..queue_len) through the atomic head counter and refill when
+    // at least `refill` of them are free; kernel_steps is then the attempt
+    // budget per system.
+    const int32_t* queue;
+    int32_t queue_len;
+    int32_t refill;
+    uint32_t* queue_head;
 };
 
 struct KernelAttr {
@@ -52,11 +61,14 @@ struct KernelAttr {
 // Lane mapping: one system per lane.  Returns hipErrorInvalidValue when no
 // instantiation exists for (ups, nk, field).
 hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream);
+// persistent launch of `waves` waves (queue mode)
+hipError_t lane_launch_persistent(const LaunchArgs& a, int block, int waves, hipStream_t stream);
 hipError_t lane_attr(int ups, int nk, int field, KernelAttr* out);
 bool lane_supported(int ups, int nk, int field);
 
 // Group mapping: next_pow2(ups) lanes per system, one wave per workgroup.
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
+hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream);
 hipError_t group_attr(int ups, int nk, int field, KernelAttr* out);
 bool group_supported(int ups, int nk, int field);
 int group_systems_per_block(int ups);

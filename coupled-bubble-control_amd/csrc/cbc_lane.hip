@@ -161,35 +161,50 @@ __device__ __forceinline__ double lane_event(const LaneSys<N, NK>& P, const doub
     return g;
 }
 
+// Per-lane integration state of one system.
+template <int N, int NK>
+struct LaneRun {
+    LaneSys<N, NK> P;
+    double y[4 * N];
+    double t, t_end, h, g_old;
+    int s, flag, attempts, rejected;
+    int didx;
+    double dnext, ddt;
+    int wcur;
+    double wend;
+    double wmax[N], wsum[N];
+};
+
+// Load system s (rkck45_coupled_numba_stepper.py:179-213, 248-358).  Returns false
+// when the system is already finished at load (its flag is written then).
 template <int N, int NK, int FIELD>
-__global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
+__device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N, NK>& R)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.ns) return;
     const size_t NS = (size_t)a.ns;
     const size_t TILE = (size_t)a.tile;
     const size_t NST = NS * TILE;
     const size_t base = (size_t)s * TILE;
-
-    // load and early termination (rkck45_coupled_numba_stepper.py:179-213)
-    double t = a.t[s];
-    const double t_end = a.t_end[s];
-    double h = a.h[s];
-    double g_old = a.event[s];
+    R.s = s;
+    R.t = a.t[s];
+    R.t_end = a.t_end[s];
+    R.h = a.h[s];
+    R.g_old = a.event[s];
+    R.flag = kActive;
+    R.attempts = 0;
+    R.rejected = 0;
     {
-        bool term = t >= t_end;
+        bool term = R.t >= R.t_end;
         int flag = term ? kSuccess : kActive;
-        if (fabs(g_old) < a.etol) {
+        if (fabs(R.g_old) < a.etol) {
             term = true;
             flag = kEventTerminal;
         }
         if (term) {
             a.flags[s] = flag;
-            return;
+            return false;
         }
     }
-
-    LaneSys<N, NK> P;
+    LaneSys<N, NK>& P = R.P;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
 #pragma unroll
@@ -208,14 +223,13 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
             P.m2[i][j] = a.cm[2 * cstride + off];
         }
     }
-    const double sp0 = a.sp[s], sp1 = a.sp[NS + s], sp2 = a.sp[2 * NS + s];
-    const double sp3 = a.sp[3 * NS + s], sp4 = a.sp[4 * NS + s];
-    P.sp0 = sp0;
-    P.sp3 = sp3;
+    P.sp0 = a.sp[s];
+    P.sp3 = a.sp[3 * NS + s];
     {
         const double TWO_PI = 6.28318530717958647692;
-        const double w0 = TWO_PI * sp1;
-        const double l0 = TWO_PI * sp2;
+        const double w0 = TWO_PI * a.sp[NS + s];
+        const double l0 = TWO_PI * a.sp[2 * NS + s];
+        const double sp4 = a.sp[4 * NS + s];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
             const double amps = a.cp[(size_t)k * NS + s];
@@ -231,220 +245,290 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
             P.d.Au[k] = amps * sp4;
         }
     }
-
-    double y[4 * N];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < N; ++i) y[j * N + i] = a.state[j * NST + base + i];
+        for (int i = 0; i < N; ++i) R.y[j * N + i] = a.state[j * NST + base + i];
 
-    g_old = lane_event<N, NK>(P, y, a.col_factor);
+    R.g_old = lane_event<N, NK>(P, R.y, a.col_factor);
 
     // dense output (rkck45_coupled_numba_stepper.py:72-112, 333-345)
-    const bool dense = a.ndo > 0;
-    int didx = 0;
-    double dnext = t, ddt = 0.0;
-    if (dense) {
+    R.didx = 0;
+    R.dnext = R.t;
+    R.ddt = 0.0;
+    if (a.ndo > 0) {
         const double tb = a.t_begin[s];
-        didx = a.dense_index[s];
-        ddt = (a.ndo == 1) ? (t_end - tb) : (t_end - tb) / (a.ndo - 1);
-        if (didx < a.ndo && dnext <= t) {
-            a.dense_time[(size_t)didx * NS + s] = t;
+        R.didx = a.dense_index[s];
+        R.ddt = (a.ndo == 1) ? (R.t_end - tb) : (R.t_end - tb) / (a.ndo - 1);
+        if (R.didx < a.ndo && R.dnext <= R.t) {
+            a.dense_time[(size_t)R.didx * NS + s] = R.t;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    a.dense_state[(((size_t)didx * 4 + j) * NS + s) * TILE + i] = y[j * N + i];
-            ++didx;
-            dnext = fmin(t + ddt, t_end);
+                    a.dense_state[(((size_t)R.didx * 4 + j) * NS + s) * TILE + i] = R.y[j * N + i];
+            ++R.didx;
+            R.dnext = fmin(R.t + R.ddt, R.t_end);
         }
     }
 
     // period windows: a window end is an intermediate end time of the clamp.
     // A launch that starts inside a window resumes its partial statistics,
     // which the previous launch left in max_r / mean_x.
-    const bool win = a.period > 0.0;
-    int wcur = 0;
-    double wend = t_end;
-    double wmax[N], wsum[N];
-    if (win) {
-        wcur = (int)floor(t / a.period + 1e-12);
-        wend = fmin((wcur + 1) * a.period, t_end);
-        const int widx = wcur - a.first_window;
-        const bool mid = (t - wcur * a.period) > 1e-9 * a.period;
-        const bool stored = widx >= 0 && widx < a.n_windows;
+    R.wcur = 0;
+    R.wend = R.t_end;
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            wmax[i] = y[i];
-            wsum[i] = 0.0;
-            if (mid && stored) {
-                wmax[i] = a.max_r[((size_t)widx * NS + s) * TILE + i];
-                wsum[i] = a.mean_x[((size_t)widx * NS + s) * TILE + i] * a.period;
+    for (int i = 0; i < N; ++i) {
+        R.wmax[i] = R.y[i];
+        R.wsum[i] = 0.0;
+    }
+    if (a.period > 0.0) {
+        R.wcur = (int)floor(R.t / a.period + 1e-12);
+        R.wend = fmin((R.wcur + 1) * a.period, R.t_end);
+        const int widx = R.wcur - a.first_window;
+        const bool mid = (R.t - R.wcur * a.period) > 1e-9 * a.period;
+        if (mid && widx >= 0 && widx < a.n_windows) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                R.wmax[i] = a.max_r[((size_t)widx * NS + s) * TILE + i];
+                R.wsum[i] = a.mean_x[((size_t)widx * NS + s) * TILE + i] * a.period;
             }
         }
     }
+    return true;
+}
 
-    int flag = kActive;
-    int attempts = 0, rejected = 0;
-    for (int64_t step = 0; step < a.kernel_steps; ++step) {
-        ++attempts;
-        // clamp (rkck45_coupled_numba_stepper.py:366-373)
-        if (h < a.min_step) h = a.min_step;
-        if (h > a.max_step) h = a.max_step;
-        {
-            const double remain = wend - t;
-            if (h > remain) h = remain;
+// One Cash--Karp attempt (rkck45_coupled_numba_stepper.py:362-677).  Returns true
+// when the system stopped (R.flag set).
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ bool lane_attempt(const LaunchArgs& a, LaneRun<N, NK>& R)
+{
+    const LaneSys<N, NK>& P = R.P;
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    const int s = R.s;
+    double& t = R.t;
+    double& h = R.h;
+    double (&y)[4 * N] = R.y;
+    ++R.attempts;
+    // clamp (rkck45_coupled_numba_stepper.py:366-373)
+    if (h < a.min_step) h = a.min_step;
+    if (h > a.max_step) h = a.max_step;
+    {
+        const double remain = R.wend - t;
+        if (h > remain) h = remain;
+    }
+
+    // Cash--Karp stages (rkck45_coupled_numba_stepper.py:379-484)
+    double k1[4 * N], k2[4 * N], k3[4 * N], k4[4 * N], k5[4 * N], k6[4 * N], tmp[4 * N];
+    lane_rhs<N, NK, FIELD>(P, t, y, k1);
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
+    lane_rhs<N, NK, FIELD>(P, t + CK::a2 * h, tmp, k2);
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
+    lane_rhs<N, NK, FIELD>(P, t + CK::a3 * h, tmp, k3);
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
+    lane_rhs<N, NK, FIELD>(P, t + CK::a4 * h, tmp, k4);
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c)
+        tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
+    lane_rhs<N, NK, FIELD>(P, t + CK::a5 * h, tmp, k5);
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c)
+        tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
+                             CK::b65 * k5[c]);
+    lane_rhs<N, NK, FIELD>(P, t + CK::a6 * h, tmp, k6);
+
+    double ynew[4 * N];
+#pragma unroll
+    for (int c = 0; c < 4 * N; ++c)
+        ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
+                              CK::c5 * k5[c] + CK::c6 * k6[c]);
+    const double g_new = lane_event<N, NK>(P, ynew, a.col_factor);
+
+    // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
+    const bool crossing = R.g_old * g_new < 0.0;
+    if (crossing) {
+        if (h <= a.min_step) {  // cannot shrink further: no progress possible
+            R.flag = kStepSizeLimit;
+            return true;
         }
-
-        // Cash--Karp stages (rkck45_coupled_numba_stepper.py:379-484)
-        double k1[4 * N], k2[4 * N], k3[4 * N], k4[4 * N], k5[4 * N], k6[4 * N], tmp[4 * N];
-        lane_rhs<N, NK, FIELD>(P, t, y, k1);
+        const double aold = fabs(R.g_old);
+        const double ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
+        h = h * ratio;
+        ++R.rejected;
+    } else {
+        // error norm (rkck45_coupled_numba_stepper.py:525-536)
+        double ratio = 1.0e30;
+        bool bad = false;
 #pragma unroll
-        for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
-        lane_rhs<N, NK, FIELD>(P, t + CK::a2 * h, tmp, k2);
-#pragma unroll
-        for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
-        lane_rhs<N, NK, FIELD>(P, t + CK::a3 * h, tmp, k3);
-#pragma unroll
-        for (int c = 0; c < 4 * N; ++c)
-            tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
-        lane_rhs<N, NK, FIELD>(P, t + CK::a4 * h, tmp, k4);
-#pragma unroll
-        for (int c = 0; c < 4 * N; ++c)
-            tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
-        lane_rhs<N, NK, FIELD>(P, t + CK::a5 * h, tmp, k5);
-#pragma unroll
-        for (int c = 0; c < 4 * N; ++c)
-            tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
-                                 CK::b65 * k5[c]);
-        lane_rhs<N, NK, FIELD>(P, t + CK::a6 * h, tmp, k6);
-
-        double ynew[4 * N];
-#pragma unroll
-        for (int c = 0; c < 4 * N; ++c)
-            ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
-                                  CK::c5 * k5[c] + CK::c6 * k6[c]);
-        const double g_new = lane_event<N, NK>(P, ynew, a.col_factor);
-
-        // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
-        const bool crossing = g_old * g_new < 0.0;
-        if (crossing) {
-            if (h <= a.min_step) {  // cannot shrink further: no progress possible
-                flag = kStepSizeLimit;
-                break;
-            }
-            const double aold = fabs(g_old);
-            const double ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
-            h = h * ratio;
-            ++rejected;
-        } else {
-            // error norm (rkck45_coupled_numba_stepper.py:525-536)
-            double ratio = 1.0e30;
-            bool bad = false;
-#pragma unroll
-            for (int c = 0; c < 4 * N; ++c) {
-                const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
-                                           CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
-                const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
-                const double r = tol * rcp_fast(e);
-                bad |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);
-                if (r < ratio) ratio = r;
-            }
-            if (bad) {
-                flag = kNaN;
-                break;
-            }
-            const bool ok = ratio >= 1.0;
-            // step size below the time resolution: an accepted step would not
-            // advance t, a rejected one cannot shrink (the reference clamps h at
-            // MIN_STEP and spins until its step budget runs out)
-            if (ok ? (t + h == t) : (h <= a.min_step)) {
-                flag = kStepSizeLimit;
-                break;
-            }
-            const double fac = step_factor(ok, ratio);
-            if (ok) {
-                if (win) {
-#pragma unroll
-                    for (int i = 0; i < N; ++i) {
-                        if (ynew[i] > wmax[i]) wmax[i] = ynew[i];
-                        wsum[i] += 0.5 * h * (y[N + i] + ynew[N + i]);
-                    }
-                }
-                t = t + h;
-#pragma unroll
-                for (int c = 0; c < 4 * N; ++c) y[c] = ynew[c];
-                g_old = g_new;
-            } else {
-                ++rejected;
-            }
-            h = h * fac;
+        for (int c = 0; c < 4 * N; ++c) {
+            const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
+                                       CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
+            const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
+            const double r = tol * rcp_fast(e);
+            bad |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);
+            if (r < ratio) ratio = r;
         }
-
-        if (dense && didx < a.ndo && dnext <= t) {
-            a.dense_time[(size_t)didx * NS + s] = t;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int i = 0; i < N; ++i)
-                    a.dense_state[(((size_t)didx * 4 + j) * NS + s) * TILE + i] = y[j * N + i];
-            ++didx;
-            dnext = fmin(t + ddt, t_end);
+        if (bad) {
+            R.flag = kNaN;
+            return true;
         }
-
-        if (win && t >= wend) {
-            const int widx = wcur - a.first_window;
-            if (widx >= 0 && widx < a.n_windows) {
+        const bool ok = ratio >= 1.0;
+        // step size below the time resolution: an accepted step would not
+        // advance t, a rejected one cannot shrink (the reference clamps h at
+        // MIN_STEP and spins until its step budget runs out)
+        if (ok ? (t + h == t) : (h <= a.min_step)) {
+            R.flag = kStepSizeLimit;
+            return true;
+        }
+        const double fac = step_factor(ok, ratio);
+        if (ok) {
+            if (a.period > 0.0) {
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax[i];
-                    a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum[i] / a.period;
+                    if (ynew[i] > R.wmax[i]) R.wmax[i] = ynew[i];
+                    R.wsum[i] += 0.5 * h * (y[N + i] + ynew[N + i]);
                 }
             }
-            ++wcur;
-            wend = fmin((wcur + 1) * a.period, t_end);
+            t = t + h;
+#pragma unroll
+            for (int c = 0; c < 4 * N; ++c) y[c] = ynew[c];
+            R.g_old = g_new;
+        } else {
+            ++R.rejected;
+        }
+        h = h * fac;
+    }
+
+    if (a.ndo > 0 && R.didx < a.ndo && R.dnext <= t) {
+        a.dense_time[(size_t)R.didx * NS + s] = t;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                a.dense_state[(((size_t)R.didx * 4 + j) * NS + s) * TILE + i] = y[j * N + i];
+        ++R.didx;
+        R.dnext = fmin(t + R.ddt, R.t_end);
+    }
+
+    if (a.period > 0.0 && t >= R.wend) {
+        const int widx = R.wcur - a.first_window;
+        if (widx >= 0 && widx < a.n_windows) {
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                wmax[i] = y[i];
-                wsum[i] = 0.0;
+                a.max_r[((size_t)widx * NS + s) * TILE + i] = R.wmax[i];
+                a.mean_x[((size_t)widx * NS + s) * TILE + i] = R.wsum[i] / a.period;
             }
         }
-
-        // termination (rkck45_coupled_numba_stepper.py:651-663)
-        if (t >= t_end) {
-            flag = kSuccess;
-            break;
-        }
-        if (fabs(g_old) < a.etol) {
-            flag = kEventTerminal;
-            break;
+        ++R.wcur;
+        R.wend = fmin((R.wcur + 1) * a.period, R.t_end);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            R.wmax[i] = y[i];
+            R.wsum[i] = 0.0;
         }
     }
 
-    if (win && flag != kSuccess) {
-        const int widx = wcur - a.first_window;
-        if ((t - wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
+    // termination (rkck45_coupled_numba_stepper.py:651-663)
+    if (t >= R.t_end) {
+        R.flag = kSuccess;
+        return true;
+    }
+    if (fabs(R.g_old) < a.etol) {
+        R.flag = kEventTerminal;
+        return true;
+    }
+    return false;
+}
+
+// Store (rkck45_coupled_numba_stepper.py:683-704); also saves an open window.
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ void lane_finish(const LaunchArgs& a, const LaneRun<N, NK>& R)
+{
+    const size_t NS = (size_t)a.ns;
+    const size_t TILE = (size_t)a.tile;
+    const size_t NST = NS * TILE;
+    const int s = R.s;
+    const size_t base = (size_t)s * TILE;
+    if (a.period > 0.0 && R.flag != kSuccess) {
+        const int widx = R.wcur - a.first_window;
+        if ((R.t - R.wcur * a.period) > 1e-9 * a.period && widx >= 0 && widx < a.n_windows) {
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                a.max_r[((size_t)widx * NS + s) * TILE + i] = wmax[i];
-                a.mean_x[((size_t)widx * NS + s) * TILE + i] = wsum[i] / a.period;
+                a.max_r[((size_t)widx * NS + s) * TILE + i] = R.wmax[i];
+                a.mean_x[((size_t)widx * NS + s) * TILE + i] = R.wsum[i] / a.period;
             }
         }
     }
-
-    // store (rkck45_coupled_numba_stepper.py:683-704)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < N; ++i) a.state[j * NST + base + i] = y[j * N + i];
-    a.t[s] = t;
-    a.h[s] = h;
-    a.event[s] = g_old;
-    a.flags[s] = flag;
-    if (dense) a.dense_index[s] = didx;
-    if (a.total_steps) a.total_steps[s] += attempts;
-    if (a.rejected_steps) a.rejected_steps[s] += rejected;
-    if (flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+        for (int i = 0; i < N; ++i) a.state[j * NST + base + i] = R.y[j * N + i];
+    a.t[s] = R.t;
+    a.h[s] = R.h;
+    a.event[s] = R.g_old;
+    a.flags[s] = R.flag;
+    if (a.ndo > 0) a.dense_index[s] = R.didx;
+    if (a.total_steps) a.total_steps[s] += R.attempts;
+    if (a.rejected_steps) a.rejected_steps[s] += R.rejected;
+    if (R.flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+}
+
+// Plain mode (queue == null): lane = system, at most kernel_steps attempts.
+// Persistent mode: each wave refills its free lanes from the work queue with one
+// atomic per refill (free lanes take consecutive queue slots in lane order) once
+// at least `refill` lanes are free, or all are; the wave exits when the queue is
+// drained and its lanes are idle.  Both modes run the same attempt loop.
+template <int N, int NK, int FIELD>
+__global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
+{
+    LaneRun<N, NK> R;
+    bool busy = false;
+    int64_t used = 0;
+    const bool persistent = a.queue != nullptr;
+    bool drained = !persistent;
+    if (!persistent) {
+        const int s = blockIdx.x * blockDim.x + threadIdx.x;
+        if (s < a.ns) busy = lane_begin<N, NK, FIELD>(a, s, R);
+    }
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        if (!drained) {
+            const uint64_t freemask = __ballot(!busy);
+            const int nfree = __popcll(freemask);
+            if (nfree > 0 && (nfree >= a.refill || nfree == 64)) {
+                const int leader = __ffsll((unsigned long long)freemask) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
+                base = __shfl(base, leader);
+                if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
+                if (!busy) {
+                    const uint32_t idx = base + (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
+                    if (idx < (uint32_t)a.queue_len) {
+                        const int q = a.queue[idx];
+                        if (q >= 0 && q < a.ns) {  // malformed entries are skipped
+                            busy = lane_begin<N, NK, FIELD>(a, q, R);
+                            used = 0;
+                        }
+                    }
+                }
+            }
+        }
+        if (!__any(busy)) {
+            if (drained) break;
+            continue;
+        }
+        if (busy) {
+            const bool stop = lane_attempt<N, NK, FIELD>(a, R);
+            if (stop || ++used >= a.kernel_steps) {
+                lane_finish<N, NK, FIELD>(a, R);
+                busy = false;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -496,6 +580,17 @@ hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream)
     if (!fn) return hipErrorInvalidValue;
     if (block <= 0 || block > 256 || (block % 64) != 0) return hipErrorInvalidValue;
     const int grid = (a.ns + block - 1) / block;
+    void* args[] = {const_cast<LaunchArgs*>(&a)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+}
+
+hipError_t lane_launch_persistent(const LaunchArgs& a, int block, int waves, hipStream_t stream)
+{
+    const void* fn = lane_pick(a.ups, a.nk, a.field);
+    if (!fn || !a.queue || !a.queue_head) return hipErrorInvalidValue;
+    if (block <= 0 || block > 256 || (block % 64) != 0 || waves <= 0) return hipErrorInvalidValue;
+    const int wpb = block / 64;
+    const int grid = (waves + wpb - 1) / wpb;
     void* args[] = {const_cast<LaunchArgs*>(&a)};
     return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
 }

@@ -141,7 +141,8 @@ const char *cbc_last_error(void);
 /* Selects and sizes the kernel for a model/execution/solver spec; replaces
  * load_coupled_kernel (backends/cupy_cuda/kernel_loader.py:9-99) and the
  * Numba make_model/make_kernel pair (models/ckm1d.py:214-232).  Allocates a
- * 16-byte device workspace (the active-system counter of cbc_solve). */
+ * 16-byte device workspace (the active-system counter of cbc_solve and the
+ * work-queue head of cbc_solve_queued). */
 int32_t cbc_plan_create(const cbc_model_spec *model, const cbc_exec_spec *exec,
                         const cbc_solver_spec *solver, cbc_plan **out);
 
@@ -171,6 +172,34 @@ int32_t cbc_solve(cbc_plan *plan, const cbc_buffers *buf, int32_t kernel_steps, 
  * max_steps bounds the attempts per system.  Asynchronous on `stream`. */
 int32_t cbc_solve_periodic(cbc_plan *plan, const cbc_buffers *buf, const cbc_period_spec *period,
                            int64_t max_steps, void *stream);
+
+/* Persistent launches over a device work queue (no counterpart in the
+ * reference, whose blocks own fixed groups of systems, rkck45_coupled_cuda_sover.cu:228-239):
+ * lanes (lane mapping) or lane groups (group mapping) take system indices from
+ * `order` (device int32[order_len], a permutation of a subset of 0..ns-1, e.g.
+ * longest horizon first) and refill as their systems stop, so uneven horizons
+ * and step counts do not leave lanes idle.  `waves` is the persistent grid in
+ * wavefronts (0 = every SIMD filled at the kernel's register occupancy);
+ * `refill` (1..64) is how many lanes (groups) of a wave must be free before the
+ * wave takes new work.  Each launch gives every system at most kernel_steps
+ * attempts; launches repeat while systems remain ACTIVE, up to
+ * ceil(max_steps/kernel_steps) launches.  Returns after the stream is
+ * synchronised; *n_launches receives the launch count. */
+int32_t cbc_solve_queued(cbc_plan *plan, const cbc_buffers *buf, const int32_t *order, int32_t order_len,
+                         int32_t waves, int32_t refill, int32_t kernel_steps, int64_t max_steps,
+                         void *stream, int32_t *n_launches);
+
+/* One asynchronous persistent launch of the queued solve (no relaunch, no
+ * synchronisation): the building block for running several plans (e.g. one
+ * per bubble count) concurrently on separate streams and for stream capture
+ * into a HIP graph (the launch sequence contains no memset node).  The number
+ * of systems left ACTIVE is read with cbc_plan_active_count. */
+int32_t cbc_launch_queued(cbc_plan *plan, const cbc_buffers *buf, const int32_t *order, int32_t order_len,
+                          int32_t waves, int32_t refill, int32_t kernel_steps, void *stream);
+
+/* Synchronises `stream` and returns the number of systems the plan's last
+ * launch left ACTIVE. */
+int32_t cbc_plan_active_count(cbc_plan *plan, void *stream, int32_t *out);
 
 #ifdef __cplusplus
 }

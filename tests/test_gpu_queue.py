@@ -1,0 +1,149 @@
+"""Persistent work-queue launches, mixed-N ensembles and graph replay.
+
+A queued launch changes only which lane integrates which system and when, never
+the arithmetic of a system, so its results must equal the plain launch bit for
+bit whatever the queue order and refill threshold."""
+import numpy as np
+import pytest
+
+from test_gpu_plain import _assert_counts, _assert_states
+
+pytestmark = pytest.mark.gpu
+
+
+def _uneven(n, ns, seed, hmax=3.0):
+    from coupledbubble_control import workloads
+
+    ws = {w.n: w for w in workloads.mixed_uneven(seed=seed, total=15 * 64)}
+    w = ws[n]
+    idx = np.arange(ns) % w.ns
+    w = w.subset(idx)
+    rng = np.random.default_rng(seed)
+    w.t_end = rng.uniform(0.2, hmax, size=ns)
+    return w
+
+
+def _snapshot(m):
+    d = m.runtime.device_buffers
+    return {k: v.detach().cpu().numpy().copy() for k, v in (
+        ("state", d.state.actual_state), ("t", d.state.actual_time), ("h", d.state.time_step),
+        ("flags", d.status.status_flags), ("steps", d.status.total_steps), ("event", d.status.actual_event))}
+
+
+@pytest.mark.parametrize("n,variant", [(2, "lane"), (2, "group"), (3, "group"), (8, "group")])
+@pytest.mark.parametrize("refill", [1, 64])
+def test_queued_equals_plain(device, n, variant, refill):
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+    from coupledbubble_control.backends.hip import rkck45_coupled_hip_solver as hs
+
+    ns = {2: 200, 3: 90, 8: 24}[n]
+    print(f"case: queued vs plain N={n} {variant} refill={refill} ns={ns}", flush=True)
+    w = _uneven(n, ns, seed=30 + n)
+    a = workloads.build_model(w, variant=KernelVariant(variant))
+    a.solve(max_steps=10 ** 7, kernel_steps=10 ** 6)
+    b = workloads.build_model(w, variant=KernelVariant(variant))
+    b._ensure_buffers()
+    b.runtime.solver = b._make_solver(b.runtime.backend, b.runtime.variant)
+    gen = torch.Generator().manual_seed(n)
+    order = torch.randperm(ns, generator=gen).to(torch.int32).cuda()
+    launches = hs.solve_queued(b.runtime.kernel, b.runtime.device_buffers, order, waves=3, refill=refill,
+                               kernel_steps=10 ** 6, max_steps=10 ** 7)
+    assert launches == 1
+    sa, sb = _snapshot(a), _snapshot(b)
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+
+
+def test_queued_budget_relaunch(device):
+    """A small per-launch budget forces relaunches; the result still equals one plain solve."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends.hip import rkck45_coupled_hip_solver as hs
+
+    print("case: queued relaunch, budget 300", flush=True)
+    w = _uneven(2, 128, seed=40)
+    a = workloads.build_model(w)
+    a.solve(max_steps=10 ** 7, kernel_steps=10 ** 6)
+    b = workloads.build_model(w)
+    b._ensure_buffers()
+    b.runtime.solver = b._make_solver(b.runtime.backend, b.runtime.variant)
+    launches = hs.solve_queued(b.runtime.kernel, b.runtime.device_buffers, kernel_steps=300, max_steps=10 ** 7)
+    assert launches > 1
+    sa, sb = _snapshot(a), _snapshot(b)
+    for k in ("state", "t", "h", "flags", "steps"):
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+
+
+def test_mixed_ensemble_against_oracle(device):
+    """Bins N = 2, 3, 4, 8 with uneven horizons solved together; each bin vs the oracle."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.ensemble import MixedEnsemble
+    from oracle import ckm1d_oracle as O
+
+    print("case: mixed ensemble 2/3/4/8", flush=True)
+    models, refs = [], []
+    for n, ns in ((2, 160), (3, 80), (4, 40), (8, 20)):
+        w = _uneven(n, ns, seed=50 + n, hmax=1.5)
+        m = workloads.build_model(w)
+        ref = O.host_buffers_dict(m)
+        cnt = O.solve(O.spec_from_model(m), ref, kernel_steps=10 ** 6, max_steps=10 ** 7)
+        models.append(m)
+        refs.append((ref, cnt))
+    ens = MixedEnsemble(models)
+    rounds = ens.solve()
+    assert rounds == 1
+    for m, (ref, cnt) in zip(models, refs):
+        m.buffer_ops.sync_to_host(m.runtime.device_buffers.state, m.runtime.host_buffers.state)
+        m.buffer_ops.sync_to_host(m.runtime.device_buffers.status, m.runtime.host_buffers.status)
+        np.testing.assert_array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+        _assert_states(m, ref, m._execution_spec.ups)
+        _assert_counts(m, cnt)
+
+
+def test_graph_replay_matches_eager(device):
+    """One launch round captured in a HIP graph and replayed three times (eager resets
+    of the buffers in between) gives the eager result every time."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.ensemble import MixedEnsemble
+
+    print("case: graph capture and 3 replays", flush=True)
+
+    def build():
+        ms = []
+        for n, ns in ((2, 96), (4, 32)):
+            ms.append(workloads.build_model(_uneven(n, ns, seed=60 + n, hmax=1.0)))
+        return ms
+
+    eager = MixedEnsemble(build())
+    eager.solve()
+    want = [_snapshot(m) for m in eager.models]
+
+    gm = build()
+    ens = MixedEnsemble(gm)
+    ens._prepare()
+    init = [{k: v.clone() for k, v in (("state", m.runtime.device_buffers.state.actual_state),
+                                         ("t", m.runtime.device_buffers.state.actual_time),
+                                         ("h", m.runtime.device_buffers.state.time_step),
+                                         ("event", m.runtime.device_buffers.status.actual_event))}
+            for m in gm]
+    ens.capture()
+    for rep in range(3):
+        for m, i0 in zip(gm, init):
+            d = m.runtime.device_buffers
+            d.state.actual_state.copy_(i0["state"])
+            d.state.actual_time.copy_(i0["t"])
+            d.state.time_step.copy_(i0["h"])
+            d.status.actual_event.copy_(i0["event"])
+            d.status.status_flags.zero_()
+            d.status.total_steps.zero_()
+        torch.cuda.synchronize()
+        ens.replay()
+        torch.cuda.synchronize()
+        for m, wnt in zip(gm, want):
+            got = _snapshot(m)
+            for k in ("state", "t", "h", "flags", "steps"):
+                np.testing.assert_array_equal(got[k], wnt[k], err_msg=f"replay {rep}: {k}")
