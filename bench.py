@@ -1,21 +1,36 @@
 """Benchmark: integrated acoustic periods per second on one MI355X.
 
-Workload (default): the parameter study -- 65 536 systems of N=2 coupled
-bubbles on a 256 x 256 (PA1, PA2) grid over [0, 2] bar, R0 = (44, 60) um,
-water at 25 degC, dual-frequency standing wave (25 + 50 kHz), FP64 Cash--Karp
-at rtol = atol = 1e-10, 200 acoustic periods with per-period max R/R0 and
-mean x recorded for periods 51..200.
+Workloads (``--workload``; water at 25 degC, FP64 Cash--Karp, Keller--Miksis
+bubbles with implicit radial/translational coupling):
 
-A step integrates every system through ``--periods-per-step`` more periods
-(one launch of the period-window kernel; state, t and h stay resident in HBM
-between steps).  ``--steps 20 --periods-per-step 10`` is the whole 200-period
-job.  Warm-up steps run on a separate copy of the ensemble and are discarded.
+* ``serving`` (default) -- the env-step serving loop: 16 384 resident systems
+  (12 288 of N=2, 4 096 of N=4, 25 + 50 kHz standing wave, rtol = atol = 1e-9).
+  Requests of 64..1024 envs (one vector env's step: new PA1, PA2 ~ U[0, 1.5] bar,
+  integrate K = 5 periods from the resident state and step size) arrive Poisson
+  at 200 000 env-steps/s; failed envs and 0.1 % of envs per step (10 % per 100
+  steps) restart with fresh radii and positions.  A step is one request;
+  ``--steps`` requests are timed.  The same requests are then served by the
+  plain path (one kernel launch per batch, per bubble count), and both paths
+  are also driven closed-loop (each vector env resubmits as soon as its step
+  completes) for ``--saturation-s`` seconds to measure capacity.
+  value = integrated periods / wall seconds of the continuous path under the
+  Poisson load; latency = request arrival to completion.
+* ``param_study`` -- 65 536 systems of N=2 on a 256 x 256 (PA1, PA2) grid over
+  [0, 2] bar, R0 = (44, 60) um, 200 periods with per-period max R/R0 and mean x
+  for periods 51..200, rtol = atol = 1e-10; a step is ``--periods-per-step``
+  more periods of every system (one launch of the window kernel).
+* ``mixed`` -- 32 768 systems with N in {2, 3, 4, 8} at 8:4:2:1 and horizons of
+  1..50 periods, rtol = atol = 1e-9: the autotuned persistent work-queue
+  ensemble against one plain launch loop per bubble count; a step is the whole
+  ensemble.
+* ``large`` -- 4 096 systems of N=16 (32 x 32 coupling solve per RHS) mixed 1:1
+  with 4 096 systems of N=2, 20 periods, rtol = atol = 1e-9; as ``mixed``.
 
-value = sum over systems of integrated periods / wall seconds of the timed
-steps.  The JSON line also carries the FP64 roofline of the kernel
-(algorithmic FLOPs per attempt times attempts, over the kernel time measured
-with HIP events on the launch stream) and a CPU baseline (the C oracle, the
-same algorithm, OpenMP over systems) timed on a bounded sample.
+Every line carries the FP64 roofline of the dominant kernel (algorithmic FLOPs
+per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
+device counters recorded, over the kernel time) and a CPU baseline: the C
+oracle (same algorithm, OpenMP over systems) timed on a bounded sample of the
+same workload.
 """
 from __future__ import annotations
 
@@ -31,7 +46,7 @@ sys.path.insert(0, str(ROOT / "coupled-bubble-control_amd"))
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
-PEAK_HBM_GBS = 8000.0
+METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
 def flops_per_attempt(n: int) -> int:
@@ -50,13 +65,17 @@ def flops_per_attempt(n: int) -> int:
 def parse():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large"], default="serving")
+    p.add_argument("--steps", type=int, default=None,
+                   help="serving: requests timed (default 1100, ~3 s of arrivals); param_study: steps of "
+                        "--periods-per-step (default 20); mixed/large: ensemble solves (default 1)")
     p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--rate", type=float, default=200000.0, help="serving: offered env-steps/s")
+    p.add_argument("--saturation-s", type=float, default=3.0, help="serving: closed-loop seconds per path (0: off)")
     p.add_argument("--periods-per-step", type=float, default=10.0)
-    p.add_argument("--side", type=int, default=256, help="PA grid side (systems = side^2)")
+    p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
-    p.add_argument("--cpu-systems", type=int, default=2048)
-    p.add_argument("--cpu-periods", type=float, default=20.0)
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample length")
     return p.parse_args()
 
 
@@ -65,24 +84,159 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    log("importing torch")
+
+def roofline(flops: float, seconds: float, **extra) -> dict:
+    achieved = flops / seconds / 1e12 if seconds > 0 else 0.0
+    out = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+    out.update(extra)
+    return out
+
+
+def attempts_of(models) -> dict:
+    import torch
+
+    return {m._execution_spec.ups: int(m.runtime.device_buffers.status.total_steps.to(torch.int64).sum().item())
+            for m in models}
+
+
+def cpu_step_sample(W, ws, target_s: float, label: str):
+    """C oracle (OpenMP) on growing subsets of the given workloads until one run takes
+    about target_s / 4, then scaled to target_s; returns the baseline dict."""
+    import numpy as np
+
+    from oracle import ckm1d_oracle as O
+
+    nthreads = cpu_threads()
+    frac = 1.0 / 512
+    while True:
+        periods, wall, systems = 0.0, 0.0, 0
+        for w in ws:
+            k = max(8, int(w.ns * frac))
+            sub = w.subset(np.linspace(0, w.ns - 1, k).astype(int))
+            m = W.build_model(sub)
+            bufs = O.host_buffers_dict(m)
+            spec = O.spec_from_model(m)
+            c0 = time.perf_counter()
+            O.solve(spec, bufs, kernel_steps=1 << 20, max_steps=1 << 22)
+            wall += time.perf_counter() - c0
+            periods += float(np.sum(bufs["actual_time"]))
+            systems += k
+        if wall >= target_s / 4 or frac >= 1.0:
+            break
+        frac = min(1.0, frac * min(8.0, max(2.0, target_s / max(wall, 1e-3))))
+    return {"value": periods / wall, "unit": "periods/s", "cores": nthreads, "kind": "port",
+            "sample": f"{systems} systems of {label} ({periods:.0f} periods), C oracle with OpenMP, {wall:.1f} s"}
+
+
+# ---------------------------------------------------------------------------
+def bench_serving(args):
+    import numpy as np
+    import torch
+
+    from coupledbubble_control import workloads as W
+    from coupledbubble_control.backends.hip.serve import ServeOptions
+    from coupledbubble_control.serving import ServingEngine, c0_scene, closed_loop, make_requests, vector_envs
+
+    sizes = {2: 12288, 4: 4096}
+    K = 5.0
+    steps = args.steps or 1100
+    venvs = vector_envs(sizes, np.random.default_rng(3))
+    lam = W.wavelength(W.water_25c(), 25.0)
+
+    def engine(mode):
+        bins_w = W.serving_bins(seed=3, sizes=sizes, periods=K)
+        bins = {n: W.build_model(w) for n, w in bins_w.items()}
+        eng = ServingEngine(bins, periods=K, mode=mode, reset_prob=0.001,
+                            reset_scene=lambda n, k, rng: c0_scene(n, k, rng, lam),
+                            serve_opts=ServeOptions(max_attempts=1 << 20), seed=3)
+        return eng, bins
+
+    def run(mode, reqs=None, closed=None):
+        eng, bins = engine(mode)
+        a0 = attempts_of(bins.values())
+        try:
+            if closed is not None:
+                first, feeder = closed_loop(venvs, closed, seed=9)
+                st = eng.run(first, timeout_s=300.0, progress=log, feeder=feeder)
+            else:
+                st = eng.run(reqs, timeout_s=300.0, progress=log)
+        finally:
+            eng.close()
+        a1 = attempts_of(bins.values())
+        st["attempts"] = {n: a1[n] - a0[n] for n in a1}
+        st["flops"] = float(sum(st["attempts"][n] * flops_per_attempt(n) for n in a1))
+        info = {n: m.runtime.kernel.info() for n, m in bins.items()}
+        return st, info
+
+    log(f"serving: {sum(sizes.values())} resident systems, {len(venvs)} vector envs, warm-up")
+    for _ in range(args.warmup):
+        run("continuous", make_requests(venvs, args.rate, count=60, seed=100))
+    torch.cuda.synchronize()
+    log(f"continuous: {steps} requests at {args.rate:g} env-steps/s")
+    cont, info = run("continuous", make_requests(venvs, args.rate, count=steps, seed=3))
+    log(f"continuous: {cont['periods_per_s']:.4g} periods/s p50 {cont['p50_ms']:.2f} ms p99 {cont['p99_ms']:.2f} ms")
+    log("plain: same requests")
+    plain, _ = run("plain", make_requests(venvs, args.rate, count=steps, seed=3))
+    log(f"plain: {plain['periods_per_s']:.4g} periods/s p50 {plain['p50_ms']:.2f} ms p99 {plain['p99_ms']:.2f} ms")
+    sat = None
+    if args.saturation_s > 0:
+        log(f"closed loop {args.saturation_s:g} s: continuous")
+        sc, _ = run("continuous", closed=args.saturation_s)
+        log(f"closed loop {args.saturation_s:g} s: plain")
+        sp, _ = run("plain", closed=args.saturation_s)
+        sat = {"continuous_periods_per_s": sc["periods_per_s"], "plain_periods_per_s": sp["periods_per_s"],
+               "ratio": sc["periods_per_s"] / sp["periods_per_s"],
+               "continuous_p50_ms": sc["p50_ms"], "continuous_p99_ms": sc["p99_ms"],
+               "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
+               "continuous_tflops": sc["flops"] / sc["wall_s"] / 1e12}
+
+    cpu = None
+    if args.cpu_baseline:
+        log("cpu baseline")
+        ws = list(W.serving_bins(seed=3, sizes=sizes, periods=K).values())
+        cpu = cpu_step_sample(W, ws, args.cpu_seconds, "the resident set, one 5-period step each")
+    keys = ("wall_s", "requests", "env_steps", "periods", "periods_per_s", "env_steps_per_s", "p50_ms", "p99_ms",
+            "mean_ms", "max_ms", "failed_env_steps", "resets")
+    return {
+        "value": cont["periods_per_s"],
+        "unit": "periods/s",
+        "steps": steps,
+        "ms_per_step": cont["wall_s"] * 1e3 / steps,
+        "config": {"workload": "serving_16384_N2x12288_N4x4096", "resident_systems": 16384,
+                   "bins": {"2": 12288, "4": 4096}, "vector_envs": len(venvs), "burst_envs": [64, 1024],
+                   "offered_env_steps_per_s": args.rate, "periods_per_step": K, "rtol": 1e-9, "atol": 1e-9,
+                   "field": "SW_N", "drive_khz": [25.0, 50.0], "reset": "failed envs + 10% per 100 steps",
+                   "mapping": {str(n): i.mapping for n, i in info.items()},
+                   "vgprs": {str(n): i.num_regs for n, i in info.items()}},
+        "latency_ms": {"p50": cont["p50_ms"], "p99": cont["p99_ms"], "plain_p50": plain["p50_ms"],
+                       "plain_p99": plain["p99_ms"]},
+        "vs_plain": cont["periods_per_s"] / plain["periods_per_s"],
+        "env_steps_per_s": cont["env_steps_per_s"],
+        "continuous": {k: cont[k] for k in keys},
+        "plain": {k: plain[k] for k in keys},
+        "saturation": sat,
+        "roofline": roofline(cont["flops"], cont["wall_s"], window="timed region (persistent serving kernels)",
+                             flops_per_attempt={str(n): flops_per_attempt(n) for n in sizes},
+                             attempts={str(n): v for n, v in cont["attempts"].items()}),
+        "cpu_baseline": cpu,
+    }
+
+
+# ---------------------------------------------------------------------------
+def bench_param_study(args):
     import numpy as np
     import torch
 
     from coupledbubble_control import workloads as W
     from coupledbubble_control.backends.hip.rkck45_coupled_hip_solver import solve_periodic
 
-    assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     P = float(args.periods_per_step)
+    steps = args.steps or 20
     total_periods = 200.0
     w = W.param_study(side=args.side, periods=total_periods, discard=50)
     ns = w.ns
@@ -90,98 +244,196 @@ def main():
 
     def make():
         m = W.build_model(w)
-        hb = m.runtime.host_buffers
-        hb.state.time_end[:] = 0.0
+        m.runtime.host_buffers.state.time_end[:] = 0.0
         m._ensure_buffers()
         m.runtime.solver = m._make_solver(m.runtime.backend, m.runtime.variant)
         max_r = torch.zeros((w.n_windows, ns, 2), dtype=torch.float64, device=dev)
-        mean_x = torch.zeros_like(max_r)
-        return m, max_r, mean_x
+        return m, max_r, torch.zeros_like(max_r)
 
     def step(m, max_r, mean_x):
         db = m.runtime.device_buffers
         db.state.time_end.add_(P).clamp_(max=total_periods)
         solve_periodic(m.runtime.kernel, db, w.period, w.first_window, max_r, mean_x)
 
-    # warm-up on a discarded copy
-    log(f"building {ns} systems")
+    log(f"param study: building {ns} systems")
     mw, rw, xw = make()
     for _ in range(args.warmup):
         step(mw, rw, xw)
     torch.cuda.synchronize()
     del mw, rw, xw
-
     log("warm-up done")
     m, max_r, mean_x = make()
     db = m.runtime.device_buffers
     stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t_before = db.state.actual_time.clone()
     att_before = db.status.total_steps.to(torch.int64).sum().item()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         ev[k][0].record(stream)
         step(m, max_r, mean_x)
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {wall:.3f} s")
+    log(f"timed {steps} steps in {wall:.3f} s")
     advanced = (db.state.actual_time - t_before).sum().item()
     attempts = db.status.total_steps.to(torch.int64).sum().item() - att_before
     kernel_ms = [a.elapsed_time(b) for a, b in ev]
     flags = db.status.status_flags.cpu().numpy()
-    value = advanced / wall
-    achieved_tflops = attempts * F / (sum(kernel_ms) * 1e-3) / 1e12
     info = m.runtime.kernel.info()
 
     cpu = None
     if args.cpu_baseline:
         from oracle import ckm1d_oracle as O
 
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sub = w.subset(np.linspace(0, ns - 1, args.cpu_systems).astype(int))
-        sub.t_end[:] = args.cpu_periods
-        ms = W.build_model(sub)
-        bufs = O.host_buffers_dict(ms)
-        spec = O.spec_from_model(ms)
-        log(f"cpu baseline: {args.cpu_systems} systems x {args.cpu_periods:g} periods on {nthreads} threads")
-        c0 = time.perf_counter()
-        out = O.solve_periodic(spec, bufs, w.period, 0, 0)
-        cwall = time.perf_counter() - c0
-        log(f"cpu baseline done in {cwall:.1f} s")
+        nthreads = cpu_threads()
+        cpu_systems, cpu_periods = 2048, 20.0
+        for _ in range(4):
+            sub = w.subset(np.linspace(0, ns - 1, cpu_systems).astype(int))
+            sub.t_end[:] = cpu_periods
+            ms = W.build_model(sub)
+            bufs = O.host_buffers_dict(ms)
+            spec = O.spec_from_model(ms)
+            log(f"cpu baseline: {cpu_systems} systems x {cpu_periods:g} periods on {nthreads} threads")
+            c0 = time.perf_counter()
+            O.solve_periodic(spec, bufs, w.period, 0, 0)
+            cwall = time.perf_counter() - c0
+            if cwall >= args.cpu_seconds / 3 or cpu_systems >= ns:
+                break
+            cpu_systems = min(ns, int(cpu_systems * min(8.0, args.cpu_seconds / max(cwall, 1e-3))))
         cpu = {"value": float(np.sum(bufs["actual_time"]) / cwall), "unit": "periods/s", "cores": nthreads,
                "kind": "port",
-               "sample": f"{args.cpu_systems} grid points x {args.cpu_periods:g} periods (same physics, "
-                         f"rtol=atol=1e-10), C oracle with OpenMP, {cwall:.1f} s",
-               "attempts": int(out["total_steps"].sum())}
-
-    line = {
-        "metric": "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain",
-        "value": value,
+               "sample": f"{cpu_systems} grid points x {cpu_periods:g} periods (same physics, rtol=atol=1e-10), "
+                         f"C oracle with OpenMP, {cwall:.1f} s"}
+    return {
+        "value": advanced / wall,
         "unit": "periods/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": w.name, "systems": ns, "bubbles_per_system": w.n,
-                   "periods_per_step": P, "periods_total": min(total_periods, P * args.steps),
-                   "rtol": w.rtol, "atol": w.atol, "field": w.field, "drive_khz": [25.0, 50.0],
+        "steps": steps,
+        "ms_per_step": wall * 1e3 / steps,
+        "config": {"workload": w.name, "systems": ns, "bubbles_per_system": w.n, "periods_per_step": P,
+                   "periods_total": min(total_periods, P * steps), "rtol": w.rtol, "atol": w.atol,
+                   "field": w.field, "drive_khz": [25.0, 50.0],
                    "window_stats": "max R/R0 and mean x per period, periods 51..200",
                    "mapping": info.mapping, "block_size": info.block_size, "vgprs": info.num_regs,
                    "scratch_bytes": info.local_bytes},
-        "roofline": {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved_tflops,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "traffic": None, "flops_per_attempt": F, "attempts": int(attempts),
-                     "kernel_ms_mean": float(np.mean(kernel_ms))},
+        "roofline": roofline(attempts * F, sum(kernel_ms) * 1e-3, window="kernel (HIP events per launch)",
+                             flops_per_attempt=F, attempts=int(attempts), kernel_ms_mean=float(np.mean(kernel_ms))),
         "cpu_baseline": cpu,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(flags, return_counts=True))},
     }
+
+
+# ---------------------------------------------------------------------------
+def bench_ensemble(args, large: bool):
+    import numpy as np
+    import torch
+
+    from coupledbubble_control import workloads as W
+    from coupledbubble_control.ensemble import MixedEnsemble, autotune
+
+    ws = W.large_coupling() if large else W.mixed_uneven()
+    name = ("large_coupling_4096xN16_4096xN2" if large
+            else "mixed_uneven_32768_" + "_".join(f"N{w.n}x{w.ns}" for w in ws))
+    steps = args.steps or 1
+
+    def models():
+        return [W.build_model(w) for w in ws]
+
+    log(f"{name}: autotune")
+    tune = autotune(models(), horizon=0.5, limit=4096)
+    tuning = {n: r.choice for n, r in tune.items()}
+    log(f"tuning: { {n: (c.variant, c.refill) for n, c in tuning.items()} }")
+
+    # warm-up: one short ensemble run
+    for _ in range(args.warmup):
+        ms = models()
+        for m in ms:
+            m.runtime.host_buffers.state.time_end[:] = np.minimum(m.runtime.host_buffers.state.time_end, 0.25)
+        MixedEnsemble(ms, tuning).solve()
+        torch.cuda.synchronize()
+
+    def timed_queued():
+        ms = models()
+        ens = MixedEnsemble(ms, tuning)
+        ens._prepare()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rounds = ens.solve()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ms, rounds
+
+    def timed_plain():
+        ms = models()
+        for m in ms:
+            m._ensure_buffers()
+            m.runtime.solver = m._make_solver(m.runtime.backend, m.runtime.variant)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for m in ms:
+            m.solve(max_steps=1 << 22, kernel_steps=1 << 20)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ms
+
+    walls, flops, periods = [], 0.0, 0.0
+    for k in range(steps):
+        log(f"queued ensemble solve {k + 1}/{steps}")
+        wall, ms, rounds = timed_queued()
+        walls.append(wall)
+        att = attempts_of(ms)
+        flops += sum(att[n] * flops_per_attempt(n) for n in att)
+        periods += float(sum(m.runtime.device_buffers.state.actual_time.sum().item() for m in ms))
+    wall = sum(walls)
+    log(f"queued: {periods / wall:.4g} periods/s")
+    log("plain: one launch loop per bubble count")
+    pwall, pms = timed_plain()
+    pperiods = float(sum(m.runtime.device_buffers.state.actual_time.sum().item() for m in pms))
+    log(f"plain: {pperiods / pwall:.4g} periods/s")
+    flags = {w.n: np.unique(m.runtime.device_buffers.status.status_flags.cpu().numpy(), return_counts=True)
+             for w, m in zip(ws, ms)}
+    cpu = None
+    if args.cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_step_sample(W, ws, args.cpu_seconds, name)
+    return {
+        "value": periods / wall,
+        "unit": "periods/s",
+        "steps": steps,
+        "ms_per_step": wall * 1e3 / steps,
+        "config": {"workload": name, "bins": {str(w.n): w.ns for w in ws}, "rtol": 1e-9, "atol": 1e-9,
+                   "field": "SW_N", "drive_khz": [25.0, 37.5] if large else [25.0, 50.0],
+                   "tuning": {str(n): {"mapping": c.variant, "refill": c.refill, "waves": c.waves}
+                              for n, c in tuning.items()}},
+        "vs_plain": (periods / wall) / (pperiods / pwall),
+        "plain_periods_per_s": pperiods / pwall,
+        "roofline": roofline(flops / steps, wall / steps, window="timed region (concurrent persistent launches)",
+                             flops_per_attempt={str(w.n): flops_per_attempt(w.n) for w in ws}),
+        "cpu_baseline": cpu,
+        "status_counts": {str(n): {str(int(a)): int(b) for a, b in zip(*f)} for n, f in flags.items()},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+    log("importing torch")
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    if args.workload == "serving":
+        out = bench_serving(args)
+    elif args.workload == "param_study":
+        out = bench_param_study(args)
+    else:
+        out = bench_ensemble(args, large=args.workload == "large")
+    line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
+            "steps": out.pop("steps"), "warmup": args.warmup, "ms_per_step": out.pop("ms_per_step"),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic"}
+    line.update(out)
     print(json.dumps(line), flush=True)
 
 

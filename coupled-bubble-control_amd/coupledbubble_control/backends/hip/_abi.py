@@ -43,6 +43,8 @@ CBC_MAP_GROUP = 2
 
 CBC_LIN_DIRECT = 0
 
+CBC_SERVE_RESET = 1
+
 FIELD_CODES = {"CONST": CBC_FIELD_CONST, "SW_N": CBC_FIELD_SW_N, "SW_A": CBC_FIELD_SW_A}
 
 _i32 = ctypes.c_int32
@@ -85,6 +87,16 @@ class PlanInfoC(ctypes.Structure):
                 ("lds_bytes", _i32), ("num_regs", _i32), ("local_bytes", _i32), ("max_threads", _i32)]
 
 
+class ServeSpecC(ctypes.Structure):
+    _fields_ = [("capacity", _i32), ("waves", _i32), ("refill", _i32), ("poll_interval", _i32),
+                ("max_attempts", _i64), ("idle_us", _i32), ("copy_state", _i32)]
+
+
+class ServeViewC(ctypes.Structure):
+    _fields_ = [("done_seq", _vp), ("out_state", _vp), ("out_misc", _vp), ("stage_state", _vp), ("stage_up", _vp),
+                ("stage_cm", _vp), ("capacity", _i32), ("waves", _i32)]
+
+
 # name -> (restype, argtypes); every entry point of include/cbc.h
 SIGNATURES = {
     "cbc_abi_version": (_i32, []),
@@ -101,6 +113,13 @@ SIGNATURES = {
                                 ctypes.POINTER(_i32)]),
     "cbc_launch_queued": (_i32, [_vp, ctypes.POINTER(BuffersC), _vp, _i32, _i32, _i32, _i32, _vp]),
     "cbc_plan_active_count": (_i32, [_vp, _vp, ctypes.POINTER(_i32)]),
+    "cbc_serve_create": (_i32, [_vp, ctypes.POINTER(BuffersC), ctypes.POINTER(ServeSpecC), _vp, ctypes.POINTER(_vp)]),
+    "cbc_serve_get_view": (_i32, [_vp, ctypes.POINTER(ServeViewC)]),
+    "cbc_serve_submit": (_i32, [_vp, _vp, _vp, _i32, _f64, _vp, _vp, _vp]),
+    "cbc_serve_poll": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "cbc_serve_stats": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_uint32),
+                               ctypes.POINTER(ctypes.c_uint32)]),
+    "cbc_serve_destroy": (_i32, [_vp]),
 }
 
 

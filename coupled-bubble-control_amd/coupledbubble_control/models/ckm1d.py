@@ -99,16 +99,33 @@ class CKM1D(Model):
     def build_parameters(self):
         """Unit parameters up[0..12], system parameters sp[0..4] and coupling
         matrices mx[0..2] of the reference (ckm1d.py:133-188)."""
-        unit_params = np.zeros(self.unit_param_shape, dtype=np.float64)
+        unit_params, coupling_matrices = self._unit_and_coupling(self._R0)
         system_params = np.zeros(self.system_param_shape, dtype=np.float64)
-        coupling_matrices = np.zeros(self.coupling_matrix_shape, dtype=np.float64)
+        TWO_PI = 2.0 * np.pi
+        m = self.mat_props
+        w_ref = TWO_PI * self._REF_FREQ
+        l_ref = m.CL / self._REF_FREQ
+        system_params[0] = 3.0 * m.PE
+        system_params[1] = 1.0 / w_ref
+        system_params[2] = l_ref / TWO_PI
+        system_params[3] = m.CL
+        system_params[4] = 1 / m.RHO / m.CL
+        return unit_params, system_params, coupling_matrices
+
+    def _unit_and_coupling(self, R0_si: np.ndarray):
+        """Unit parameters (13, k, TILE) and coupling matrices (3, UPS, k, TILE) for k systems
+        with equilibrium radii R0_si (k, TILE) in metres (ckm1d.py:146-188)."""
+        k = R0_si.shape[0]
+        es = self._execution_spec
+        UPS = es.ups
+        unit_params = np.zeros((self._model_spec.nup, k, es.tile), dtype=np.float64)
+        coupling_matrices = np.zeros((self._model_spec.ncm, UPS, k, es.tile), dtype=np.float64)
         TWO_PI = 2.0 * np.pi
         m = self.mat_props
         PV, RHO, ST, VIS, CL, P0, PE = m.PV, m.RHO, m.ST, m.VIS, m.CL, m.P0, m.PE
-        UPS = self._execution_spec.ups
         w_ref = TWO_PI * self._REF_FREQ
         l_ref = CL / self._REF_FREQ
-        R0 = self._R0[:, :UPS]
+        R0 = R0_si[:, :UPS]
 
         # Keller--Miksis
         unit_params[0, :, :UPS] = (2.0 * ST / R0 + P0 - PV) * (TWO_PI / R0 / w_ref) ** 2.0 / RHO
@@ -125,12 +142,6 @@ class CKM1D(Model):
         unit_params[10, :, :UPS] = 2 * TWO_PI / 3 * R0 ** 3
         unit_params[11, :, :UPS] = 6 * TWO_PI * VIS * R0
         unit_params[12, :, :UPS] = R0 / l_ref
-        # shared parameters
-        system_params[0] = 3.0 * PE
-        system_params[1] = 1.0 / w_ref
-        system_params[2] = l_ref / TWO_PI
-        system_params[3] = CL
-        system_params[4] = 1 / RHO / CL
         # coupling matrices: [c][row j][s][col i] = effect of bubble j on bubble i
         for row in range(UPS):
             Rj = R0[:, row:row + 1]
@@ -139,7 +150,48 @@ class CKM1D(Model):
             coupling_matrices[2, row, :, :UPS] = (18 * VIS / RHO) * (TWO_PI / w_ref) * (Rj / l_ref) ** 3 / (R0 ** 2)
         idx = np.arange(UPS)
         coupling_matrices[:, idx, :, idx] = 0.0
-        return unit_params, system_params, coupling_matrices
+        return unit_params, coupling_matrices
+
+    def scene_rows(self, idx, R0: Optional[NDArray[np.float64]], X0: NDArray[np.float64]):
+        """Rows of new episodes for systems ``idx``: new equilibrium radii R0 (k, UPS)
+        [micron] (None keeps them; the host copy of R0 is updated), positions X0 (k, UPS)
+        [wavelengths], R/R0 = 1, U = V = 0.  Returns (state (4, k, TILE), unit parameters
+        (13, k, TILE), coupling (3, UPS, k, TILE))."""
+        idx = np.asarray(idx, dtype=np.int64)
+        k = idx.size
+        es = self._execution_spec
+        ups = es.ups
+        X0 = np.asarray(X0, dtype=np.float64).reshape(k, ups)
+        if R0 is not None:
+            self._R0[idx, :ups] = np.asarray(R0, dtype=np.float64).reshape(k, ups) * 1e-6
+        up, cm = self._unit_and_coupling(self._R0[idx])
+        state = np.zeros((4, k, es.tile), dtype=np.float64)
+        state[0, :, :ups] = 1.0
+        state[1, :, :ups] = X0
+        return state, up, cm
+
+    def reset_systems(self, idx, R0: Optional[NDArray[np.float64]], X0: NDArray[np.float64],
+                      h0: float = 1e-6, stream=None) -> None:
+        """Start new episodes for systems ``idx`` (rows as in ``scene_rows``), with time 0,
+        step size h0 and event value -1; the device rows are rewritten on ``stream``
+        (reset_envs, envs/bubble_1D_pos_control.py:583-614, which keeps R0; the serving
+        workload also draws new radii).  Asynchronous on ``stream``."""
+        import torch
+
+        idx = np.asarray(idx, dtype=np.int64)
+        state, up, cm = self.scene_rows(idx, R0, X0)
+        self._ensure_buffers()
+        db = self.runtime.device_buffers
+        dev = db.state.actual_state.device
+        st = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.stream(st):
+            ti = torch.from_numpy(idx).to(dev)
+            db.params.unit_params.index_copy_(1, ti, torch.from_numpy(up).to(dev))
+            db.params.coupling_matrices.index_copy_(2, ti, torch.from_numpy(cm).to(dev))
+            db.state.actual_state.index_copy_(1, ti, torch.from_numpy(state).to(dev))
+            db.state.actual_time.index_fill_(0, ti, 0.0)
+            db.state.time_step.index_fill_(0, ti, float(h0))
+            db.status.actual_event.index_fill_(0, ti, -1.0)
 
     def build_control_parameters(self):
         """[PA (Pa), omega (rad/s), phase (rad), wave number (rad/m)] x NK (ckm1d.py:190-202)."""

@@ -1,0 +1,459 @@
+"""Serving loop for vector-environment steps (continuous batching).
+
+In the reference an environment step is a blocking call: write the actions
+into the control parameters (envs/bubble_1D_pos_control.py:731-749), run
+``model.solve`` over *all* envs of the vector env (``step_physics``, :751-753),
+then reset time, step size and event value of every env (``advance_time``,
+:815-826); terminated envs get new scenes (``reset_envs``, :583-614).
+
+Here many vector envs, of different bubble counts, share one GPU.  Every env
+is a system resident in HBM: state and last accepted step size stay in the
+device buffers between steps.  A request is one vector env's step: new drive
+amplitudes for its envs and K acoustic periods to integrate.
+
+* ``Admission`` (host logic only): requests wait in arrival order; a request
+  is admitted when the systems in flight stay within the budget (the memory
+  budget divided by the bytes one in-flight system holds), and only after the
+  previous step of the same vector env has finished.  A request that does not
+  fit the budget holds back the ones behind it (no starvation of big bursts).
+* ``mode="continuous"``: one persistent serving kernel per bubble count
+  (``backends.hip.serve.ServeQueue``); admitted requests are appended to its
+  work ring at once and waves with free lanes pick them up while other
+  requests are still integrating.
+* ``mode="plain"``: one launch per batch, per bubble count: when a bin's
+  previous launch has finished, every admitted request of that bin is written
+  into the device buffers and one kernel launch integrates them; all complete
+  together when the launch ends (the reference's blocking ``env.step``).
+* After a request completes, its envs that did not end with SUCCESS (as in
+  ``check_termination``, :771-789) and, with probability ``reset_prob`` per
+  step, any of its envs start new episodes with fresh radii and positions.
+
+A system's arithmetic does not depend on which other systems run beside it,
+so a request's results are the same alone, in the mix, and in either mode.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Callable, Deque, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+ITEM_BYTES = 48             # one work-ring item
+MISC_BYTES = 32             # tau, h, flag, attempts of a finished item
+
+
+def inflight_bytes(ups: int, tile: int) -> int:
+    """Bytes one in-flight system holds beyond its resident rows: its ring item, its
+    host-mapped end-state copy (4 x TILE doubles) and its completion record."""
+    return ITEM_BYTES + MISC_BYTES + 4 * tile * 8
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int                       # bubble count of the vector env
+    venv: int                    # vector-env id (its steps are served in order)
+    env_ids: np.ndarray          # system indices within the bin of that bubble count
+    actions: np.ndarray          # (NK, len(env_ids)) drive amplitudes [bar]
+    arrival: float               # seconds after the start of the run
+    start: float = -1.0
+    done: float = -1.0
+    flags: Optional[np.ndarray] = None
+    periods: float = 0.0
+    seq: Optional[np.ndarray] = None
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+    @property
+    def size(self) -> int:
+        return int(len(self.env_ids))
+
+
+class Admission:
+    """Arrival-ordered admission under a budget of systems in flight."""
+
+    def __init__(self, max_inflight: int):
+        if max_inflight < 1:
+            raise ValueError("max_inflight must be >= 1")
+        self.max_inflight = int(max_inflight)
+        self.pending: Deque[Request] = deque()
+        self.inflight = 0
+        self.busy_venvs: set = set()
+
+    def push(self, r: Request) -> None:
+        if r.size > self.max_inflight:
+            raise ValueError(f"request {r.rid} ({r.size} envs) can never fit the budget of {self.max_inflight}")
+        self.pending.append(r)
+
+    def admit(self, only_bin: Optional[int] = None) -> List[Request]:
+        """Remove and return the requests that may start now (optionally of one bin)."""
+        out: List[Request] = []
+        keep: Deque[Request] = deque()
+        blocked = set(self.busy_venvs)
+        room = self.max_inflight - self.inflight
+        stop = False
+        while self.pending:
+            r = self.pending.popleft()
+            if stop or r.venv in blocked or (only_bin is not None and r.n != only_bin):
+                blocked.add(r.venv)
+                keep.append(r)
+                continue
+            if r.size > room:
+                stop = True
+                keep.append(r)
+                continue
+            room -= r.size
+            blocked.add(r.venv)
+            self.busy_venvs.add(r.venv)
+            out.append(r)
+        self.pending = keep
+        self.inflight = self.max_inflight - room
+        return out
+
+    def finish(self, r: Request) -> None:
+        self.inflight -= r.size
+        self.busy_venvs.discard(r.venv)
+
+
+def c0_scene(n: int, k: int, rng, lam: float) -> Tuple[np.ndarray, np.ndarray]:
+    """New episodes as in the plain CPU workload: R0 ~ U[20, 80] um, positions sorted
+    U[0, 0.25] wavelengths with gaps >= 3 max(R0)."""
+    from .workloads import sample_positions
+
+    R0 = rng.uniform(20.0, 80.0, size=(k, n))
+    gap = 3.0 * R0.max(axis=1) * 1e-6 / lam
+    X0 = sample_positions(rng, k, n, 0.0, 0.25, gap)
+    return R0, X0
+
+
+class ServingEngine:
+    """Serves env-step requests over resident CKM1D bins (one model per bubble count)."""
+
+    def __init__(self, bins: Dict[int, object], periods: float = 5.0, mode: str = "continuous",
+                 budget_bytes: int = 64 << 20, serve_opts=None, reset_prob: float = 0.0,
+                 reset_scene: Optional[Callable] = None, seed: int = 0, keep_step: bool = True,
+                 plain_kernel_steps: int = 1 << 20):
+        import torch
+
+        from .backends.hip import rkck45_coupled_hip_solver as hs
+        from .backends.hip.serve import ServeOptions, ServeQueue
+
+        if mode not in ("continuous", "plain"):
+            raise ValueError("mode must be 'continuous' or 'plain'")
+        self.torch = torch
+        self.hs = hs
+        self.bins = dict(bins)
+        self.periods = float(periods)
+        self.mode = mode
+        self.keep_step = keep_step
+        self.reset_prob = float(reset_prob)
+        self.reset_scene = reset_scene
+        self.rng = np.random.default_rng(seed)
+        self.plain_kernel_steps = int(plain_kernel_steps)
+        per_sys = max(inflight_bytes(m._execution_spec.ups, m._execution_spec.tile) for m in self.bins.values())
+        self.admission = Admission(max(1, int(budget_bytes) // per_sys))
+        self.queues: Dict[int, object] = {}
+        self.streams: Dict[int, object] = {}
+        self.pending_reset = {n: np.zeros(m._execution_spec.ns, dtype=bool) for n, m in self.bins.items()}
+        for n, m in self.bins.items():
+            m._ensure_buffers()
+            if m.runtime.solver is None:
+                m.runtime.solver = m._make_solver(m.runtime.backend, m.runtime.variant)
+            self.streams[n] = torch.cuda.Stream()
+            if mode == "continuous":
+                o = serve_opts.get(n) if isinstance(serve_opts, dict) else serve_opts
+                o = o or ServeOptions()
+                # ring slots are freed in submission order, so finished items can wait
+                # behind a long-running one: four slots per system that can be in flight
+                cap = 64
+                while cap < 4 * min(self.admission.max_inflight, m._execution_spec.ns):
+                    cap *= 2
+                o = ServeOptions(**{**o.__dict__, "capacity": min(max(o.capacity, cap), 1 << 24)})
+                self.queues[n] = ServeQueue(m, o)
+            else:
+                # a plain launch integrates every system with actual_time < time_end:
+                # systems outside the batch are parked with time_end = actual_time
+                db = m.runtime.device_buffers
+                db.state.time_end.copy_(db.state.actual_time)
+        torch.cuda.synchronize()
+        self.inflight: List[Request] = []
+        self.batches: Dict[int, Tuple[object, List[Request]]] = {}
+        self.completed: List[Request] = []
+        self.resets = 0
+        self.ring_full = 0
+        self._t0 = 0.0
+        self._feeder = None
+
+    # -- helpers ---------------------------------------------------------------
+    def _now(self) -> float:
+        return time.perf_counter() - self._t0
+
+    def _h0(self) -> float:
+        return 0.0 if self.keep_step else 1e-6
+
+    def _finish(self, r: Request, flags: np.ndarray, t_reached: np.ndarray, now: float) -> None:
+        r.flags = flags.astype(np.int32)
+        r.periods = float(t_reached.sum())
+        r.done = now
+        self.admission.finish(r)
+        self.completed.append(r)
+        self._reset_after(r)
+        if self._feeder is not None:
+            for nr in self._feeder(r, now):
+                self.admission.push(nr)
+
+    def _reset_after(self, r: Request) -> None:
+        if self.reset_scene is None:
+            return
+        mask = r.flags != 1
+        if self.reset_prob > 0.0:
+            mask |= self.rng.random(r.size) < self.reset_prob
+        if not mask.any():
+            return
+        ids = np.asarray(r.env_ids)[mask]
+        m = self.bins[r.n]
+        R0, X0 = self.reset_scene(r.n, ids.size, self.rng)
+        if self.mode == "continuous":
+            # no device work beside the serving kernels: the new episode's rows are
+            # staged in host memory and taken by the env's next item
+            state, up, cm = m.scene_rows(ids, R0, X0)
+            self.queues[r.n].stage(ids, state, up, cm)
+            self.pending_reset[r.n][ids] = True
+        else:
+            st = self.streams[r.n]
+            m.reset_systems(ids, R0, X0, h0=1e-6, stream=st)
+            with self.torch.cuda.stream(st):
+                ti = self.torch.from_numpy(ids.astype(np.int64)).to(m.runtime.device_buffers.state.time_end.device)
+                m.runtime.device_buffers.state.time_end.index_fill_(0, ti, 0.0)
+        self.resets += int(ids.size)
+
+    # -- continuous ------------------------------------------------------------
+    def _submit(self, reqs: List[Request], now: float) -> None:
+        by_bin: Dict[int, List[Request]] = {}
+        for r in reqs:
+            by_bin.setdefault(r.n, []).append(r)
+        for n, rs in by_bin.items():
+            q = self.queues[n]
+            ids = np.concatenate([r.env_ids for r in rs]).astype(np.int32)
+            pa = np.concatenate([r.actions for r in rs], axis=1) * 1.0e5
+            reset = self.pending_reset[n][ids]
+            h0 = np.where(reset, 1e-6, self._h0())
+            seq = q.submit(ids, pa, self.periods, h0, reset if reset.any() else None)
+            if seq is None:
+                # ring slots still held by finished items behind a long one: retry later,
+                # keeping arrival order
+                for r in reversed(rs):
+                    self.admission.finish(r)
+                    self.admission.pending.appendleft(r)
+                self.ring_full += 1
+                continue
+            self.pending_reset[n][ids] = False
+            off = 0
+            for r in rs:
+                r.seq = seq[off:off + r.size]
+                r.start = now
+                off += r.size
+            self.inflight.extend(rs)
+
+    def _collect(self, now: float) -> int:
+        keep, n_done = [], 0
+        for r in self.inflight:
+            q = self.queues[r.n]
+            if q.done(r.env_ids, r.seq).all():
+                _, misc = q.results(r.env_ids)
+                self._finish(r, misc[:, 2], misc[:, 0], now)
+                n_done += 1
+            else:
+                keep.append(r)
+        self.inflight = keep
+        return n_done
+
+    # -- plain -------------------------------------------------------------------
+    def _launch_batch(self, n: int, reqs: List[Request], now: float) -> None:
+        torch = self.torch
+        m = self.bins[n]
+        db = m.runtime.device_buffers
+        st = self.streams[n]
+        nk = m._model_spec.nk
+        ids = np.concatenate([r.env_ids for r in reqs]).astype(np.int64)
+        pa = np.concatenate([r.actions for r in reqs], axis=1) * 1.0e5
+        dev = db.state.actual_state.device
+        with torch.cuda.stream(st):
+            ti = torch.from_numpy(ids).to(dev)
+            a = torch.from_numpy(np.ascontiguousarray(pa)).to(dev)
+            cp = db.params.control_params
+            for k in range(nk):
+                cp[k].index_copy_(0, ti, a[k])
+            db.state.actual_time.index_fill_(0, ti, 0.0)
+            db.state.time_end.index_fill_(0, ti, self.periods)
+            db.status.actual_event.index_fill_(0, ti, -1.0)
+            db.status.status_flags.index_fill_(0, ti, 0)
+            if not self.keep_step:
+                db.state.time_step.index_fill_(0, ti, 1e-6)
+            self.hs.launch(m.runtime.kernel, db, self.plain_kernel_steps, stream=st)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        for r in reqs:
+            r.start = now
+        self.batches[n] = (ev, reqs, ti)
+
+    def _collect_plain(self, now: float, block: bool) -> int:
+        n_done = 0
+        for n in list(self.batches):
+            ev, reqs, ti = self.batches[n]
+            if not ev.query():
+                if not block:
+                    continue
+                ev.synchronize()
+                now = self._now()
+            db = self.bins[n].runtime.device_buffers
+            with self.torch.cuda.stream(self.streams[n]):
+                flags = db.status.status_flags.index_select(0, ti).cpu().numpy()
+                t_now = db.state.actual_time.index_select(0, ti)
+                t = t_now.cpu().numpy()
+                db.state.time_end.index_copy_(0, ti, t_now)   # park the batch again
+            del self.batches[n]
+            off = 0
+            for r in reqs:
+                self._finish(r, flags[off:off + r.size], t[off:off + r.size], now)
+                off += r.size
+                n_done += 1
+            block = False
+        return n_done
+
+    # -- main loop -------------------------------------------------------------------
+    def run(self, requests: Sequence[Request], timeout_s: float = 600.0,
+            progress: Optional[Callable[[str], None]] = None,
+            feeder: Optional[Callable[[Request, float], List[Request]]] = None) -> Dict[str, float]:
+        """Serve the requests (arrival times relative to the start of the run) until all
+        have completed; ``feeder(done_request, now)`` may return follow-up requests, which
+        arrive at once (a closed loop).  Returns throughput and latency statistics."""
+        arrivals = deque(sorted(requests, key=lambda r: r.arrival))
+        self.completed = []
+        self._feeder = feeder
+        self._t0 = time.perf_counter()
+        last_report = 0.0
+        while arrivals or self.admission.pending or self.inflight or self.batches:
+            now = self._now()
+            if now > timeout_s:
+                raise TimeoutError(f"serving run exceeded {timeout_s} s ({len(self.completed)} requests done)")
+            moved = 0
+            while arrivals and arrivals[0].arrival <= now:
+                self.admission.push(arrivals.popleft())
+                moved += 1
+            if self.mode == "continuous":
+                done = self._collect(now)
+                adm = self.admission.admit()
+                if adm:
+                    self._submit(adm, now)
+                for q in self.queues.values():
+                    q.poll()
+            else:
+                done = self._collect_plain(now, block=False)
+                adm = []
+                for n in self.bins:
+                    if n not in self.batches:
+                        a = self.admission.admit(only_bin=n)
+                        if a:
+                            self._launch_batch(n, a, now)
+                            adm += a
+            if not (moved or done or adm):
+                time.sleep(2e-5)
+            if progress and now - last_report > 5.0:
+                last_report = now
+                progress(f"serving[{self.mode}] t={now:.1f}s done {len(self.completed)}")
+        wall = self._now()
+        self._feeder = None
+        return self.summary(wall)
+
+    def summary(self, wall: float) -> Dict[str, float]:
+        lat = np.array([r.latency for r in self.completed]) * 1e3
+        env_steps = sum(r.size for r in self.completed)
+        periods = sum(r.periods for r in self.completed)
+        failed = sum(int((r.flags != 1).sum()) for r in self.completed)
+        out = {"mode": self.mode, "wall_s": wall, "requests": len(self.completed), "env_steps": env_steps,
+               "periods": periods, "periods_per_s": periods / wall, "env_steps_per_s": env_steps / wall,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "failed_env_steps": failed,
+               "resets": self.resets}
+        if self.mode == "continuous":
+            out["ring_full_retries"] = self.ring_full
+            out["kernel_launches"] = int(sum(q.stats()[0] for q in self.queues.values()))
+        return out
+
+    def close(self) -> None:
+        for q in self.queues.values():
+            q.close()
+        self.queues = {}
+
+
+def vector_envs(bin_sizes: Dict[int, int], rng, lo: int = 64, hi: int = 1024) -> List[Tuple[int, np.ndarray]]:
+    """Partition each bin's systems into vector envs of integers(lo, hi + 1) envs (the last
+    one of a bin takes the remainder, merged into its predecessor if below lo)."""
+    out = []
+    for n, ns in bin_sizes.items():
+        sizes = []
+        left = ns
+        while left > 0:
+            k = int(rng.integers(lo, hi + 1))
+            k = min(k, left)
+            sizes.append(k)
+            left -= k
+        if len(sizes) > 1 and sizes[-1] < lo:
+            last = sizes.pop()
+            sizes[-1] += last
+        off = 0
+        for k in sizes:
+            out.append((n, np.arange(off, off + k, dtype=np.int32)))
+            off += k
+    return out
+
+
+def make_requests(venvs: Sequence[Tuple[int, np.ndarray]], rate_env_steps: float, duration: Optional[float] = None,
+                  seed: int = 0, pa_max: float = 1.5, nk: int = 2, count: Optional[int] = None) -> List[Request]:
+    """Poisson arrivals of vector-env steps at ``rate_env_steps`` env-steps/s in aggregate
+    (bursts at rate / mean vector-env size per second) for ``duration`` seconds or
+    ``count`` requests; the vector envs take turns in a shuffled round-robin order;
+    actions ~ U[0, pa_max] bar."""
+    if (duration is None) == (count is None):
+        raise ValueError("give exactly one of duration and count")
+    rng = np.random.default_rng(seed)
+    mean_size = float(np.mean([len(ids) for _, ids in venvs]))
+    lam = rate_env_steps / mean_size
+    order = rng.permutation(len(venvs))
+    t, j, out = 0.0, 0, []
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if (duration is not None and t > duration) or (count is not None and len(out) >= count):
+            break
+        v = int(order[j % len(order)])
+        j += 1
+        n, ids = venvs[v]
+        out.append(Request(len(out), n, v, ids, rng.uniform(0.0, pa_max, size=(nk, len(ids))), t))
+    return out
+
+
+def closed_loop(venvs: Sequence[Tuple[int, np.ndarray]], duration: float, seed: int = 0, pa_max: float = 1.5,
+                nk: int = 2):
+    """Requests for a closed loop: every vector env submits at t = 0 and, until ``duration``
+    seconds have passed, its next step as soon as the previous one completes (the
+    blocking env.step of a PPO rollout worker).  Returns (initial requests, feeder)."""
+    rng = np.random.default_rng(seed)
+    counter = [0]
+
+    def new(v: int, t: float) -> Request:
+        n, ids = venvs[v]
+        r = Request(counter[0], n, v, ids, rng.uniform(0.0, pa_max, size=(nk, len(ids))), t)
+        counter[0] += 1
+        return r
+
+    first = [new(v, 0.0) for v in range(len(venvs))]
+
+    def feeder(done: Request, now: float) -> List[Request]:
+        return [new(done.venv, now)] if now < duration else []
+
+    return first, feeder

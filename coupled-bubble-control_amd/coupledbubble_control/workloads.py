@@ -11,7 +11,8 @@ reference frequency, and per-system horizons in acoustic periods.
                      [0, 2] bar, R0 = (44, 60) um, 200 periods with the first
                      50 discarded from the per-period statistics, rtol=atol=1e-10.
 * ``mixed_uneven``   32 768 systems with N in {2, 3, 4, 8} at 8:4:2:1, uneven horizons.
-* ``serving``        16 384 resident systems, N in {2, 4} at 3:1.
+* ``serving_bins``   16 384 resident systems, N in {2, 4} at 3:1, c0-style scenes,
+                     5-period steps (the serving loop's envs).
 * ``large_coupling`` 4 096 systems of N=16 mixed 1:1 with N=2 systems.
 Water at 25 degC throughout.
 """
@@ -127,6 +128,26 @@ def mixed_uneven(seed: int = 2, total: int = 32768) -> List[Workload]:
         FR = np.tile(np.array([[25.0], [50.0]]), (1, ns))
         t_end = rng.integers(1, 51, size=ns).astype(np.float64)
         out.append(Workload(f"mixed_N{n}", n, R0, X0, PA, FR, np.zeros((2, ns)), t_end, mat=mat))
+    return out
+
+
+def serving_bins(seed: int = 3, sizes: Optional[dict] = None, periods: float = 5.0) -> dict:
+    """Resident envs of the serving workload, one Workload per bubble count (default
+    12 288 systems of N=2 and 4 096 of N=4): R0 ~ U[20, 80] um, positions sorted
+    U[0, 0.25] wavelengths with gaps >= 3 max(R0), drive 25/50 kHz with
+    PA ~ U[0, 1.5] bar, K = ``periods`` per step, rtol = atol = 1e-9."""
+    rng = np.random.default_rng(seed)
+    mat = water_25c()
+    lam = wavelength(mat, 25.0)
+    sizes = sizes or {2: 12288, 4: 4096}
+    out = {}
+    for n, ns in sizes.items():
+        R0 = rng.uniform(20.0, 80.0, size=(ns, n))
+        gap = 3.0 * R0.max(axis=1) * 1e-6 / lam
+        X0 = sample_positions(rng, ns, n, 0.0, 0.25, gap)
+        PA = rng.uniform(0.0, 1.5, size=(2, ns))
+        FR = np.tile(np.array([[25.0], [50.0]]), (1, ns))
+        out[n] = Workload(f"serving_N{n}", n, R0, X0, PA, FR, np.zeros((2, ns)), np.full(ns, periods), mat=mat)
     return out
 
 

@@ -6,6 +6,7 @@
 // becomes cbc_solve.  No exception crosses this boundary.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -392,6 +393,328 @@ int32_t cbc_solve_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t* 
         if (*plan->h_active == 0) break;
     }
     if (n_launches) *n_launches = launches;
+    return CBC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving: a persistent kernel fed from a host-mapped work ring
+// ---------------------------------------------------------------------------
+struct cbc_serve {
+    cbc_plan* plan;
+    cbc::LaunchArgs args;
+    hipStream_t stream;
+    bool own_stream;             // created here
+    hipEvent_t ended;            // recorded after each launch
+    bool launched;
+    int32_t waves;
+    int32_t capacity;
+    int32_t nk;
+    cbc::ServeItem* ring;        // host-mapped, capacity entries
+    uint32_t* ctl;               // host-mapped: [0] tail, [1] stop
+    uint32_t* done_seq;          // host-mapped (NS)
+    double* out_state;           // host-mapped (4, NS, TILE) or null
+    double* out_misc;            // host-mapped (NS, 4)
+    double* stage_state;         // host-mapped staging mirrors for episode resets
+    double* stage_up;
+    double* stage_cm;
+    uint32_t* last_seq;          // host (NS): seq of the last item submitted per system
+    uint32_t tail;               // items published
+    uint32_t retired;            // oldest ring slot not yet known finished
+    int64_t launches;
+};
+
+namespace {
+
+bool serve_running(cbc_serve* s)
+{
+    if (!s->launched) return false;
+    return hipEventQuery(s->ended) == hipErrorNotReady;
+}
+
+void serve_retire(cbc_serve* s)
+{
+    while (s->retired != s->tail) {
+        const cbc::ServeItem& it = s->ring[s->retired & (uint32_t)(s->capacity - 1)];
+        const uint32_t done = __atomic_load_n(&s->done_seq[it.sys], __ATOMIC_ACQUIRE);
+        if ((int32_t)(done - it.seq) < 0) break;
+        ++s->retired;
+    }
+}
+
+int32_t serve_launch(cbc_serve* s)
+{
+    cbc_plan* p = s->plan;
+    hipError_t e = p->mapping == CBC_MAP_LANE ? cbc::lane_launch_persistent(s->args, p->block, s->waves, s->stream)
+                                              : cbc::group_launch_persistent(s->args, s->waves, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "serving kernel launch");
+    e = hipEventRecord(s->ended, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    s->launched = true;
+    ++s->launches;
+    return CBC_OK;
+}
+
+void serve_free(cbc_serve* s)
+{
+    if (s->ended) (void)hipEventDestroy(s->ended);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->args.queue_head) (void)hipFree(s->args.queue_head);
+    if (s->ring) (void)hipHostFree(s->ring);
+    if (s->ctl) (void)hipHostFree(s->ctl);
+    if (s->done_seq) (void)hipHostFree(s->done_seq);
+    if (s->out_state) (void)hipHostFree(s->out_state);
+    if (s->out_misc) (void)hipHostFree(s->out_misc);
+    if (s->stage_state) (void)hipHostFree(s->stage_state);
+    if (s->stage_up) (void)hipHostFree(s->stage_up);
+    if (s->stage_cm) (void)hipHostFree(s->stage_cm);
+    delete[] s->last_seq;
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cbc_serve_create(cbc_plan* plan, const cbc_buffers* buf, const cbc_serve_spec* spec, void* stream,
+                         cbc_serve** out)
+{
+    try {
+        if (!plan || !spec || !out) return fail(CBC_ERR_INVALID, "cbc_serve_create: null argument");
+        *out = nullptr;
+        const int32_t cap = spec->capacity;
+        if (cap < 64 || cap > (1 << 24) || (cap & (cap - 1)) != 0)
+            return fail(CBC_ERR_INVALID, "capacity must be a power of two in [64, 2^24], got %d", cap);
+        if (spec->refill < 1 || spec->refill > 64)
+            return fail(CBC_ERR_INVALID, "refill must be in [1, 64], got %d", spec->refill);
+        if (spec->poll_interval < 1) return fail(CBC_ERR_INVALID, "poll_interval must be >= 1");
+        if (spec->max_attempts < 1) return fail(CBC_ERR_INVALID, "max_attempts must be >= 1");
+        if (spec->idle_us < 1 || spec->idle_us > 10000000)
+            return fail(CBC_ERR_INVALID, "idle_us must be in [1, 1e7], got %d", spec->idle_us);
+        if (spec->waves < 0) return fail(CBC_ERR_INVALID, "waves must be >= 0");
+        if (plan->model.nk > 2) return fail(CBC_ERR_UNSUPPORTED, "serving items carry at most 2 drive amplitudes");
+        if (plan->exec.ndo > 0) return fail(CBC_ERR_UNSUPPORTED, "serving does not record dense output");
+
+        cbc_serve* s = new (std::nothrow) cbc_serve();
+        if (!s) return fail(CBC_ERR_NOMEM, "cbc_serve_create: out of host memory");
+        int32_t rc = fill_args(plan, buf, s->args);
+        if (rc != CBC_OK) {
+            delete s;
+            return rc;
+        }
+        s->plan = plan;
+        s->stream = (hipStream_t)stream;
+        if (!s->stream) {
+            hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                s->stream = nullptr;
+                serve_free(s);
+                return hip_fail(e, "serving stream creation");
+            }
+            s->own_stream = true;
+        }
+        s->capacity = cap;
+        s->nk = plan->model.nk;
+        const size_t ns = (size_t)plan->exec.ns;
+        s->last_seq = new (std::nothrow) uint32_t[ns]();
+        if (!s->last_seq) {
+            serve_free(s);
+            return fail(CBC_ERR_NOMEM, "cbc_serve_create: out of host memory");
+        }
+        const unsigned hf = hipHostMallocCoherent | hipHostMallocMapped;
+        hipError_t e = hipHostMalloc((void**)&s->ring, sizeof(cbc::ServeItem) * (size_t)cap, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->ctl, 64, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->done_seq, sizeof(uint32_t) * ns, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->out_misc, sizeof(double) * 4 * ns, hf);
+        const size_t nst = ns * (size_t)plan->exec.tile;
+        const size_t n_cm = 3 * (size_t)plan->exec.ups * nst;
+        if (e == hipSuccess && spec->copy_state) e = hipHostMalloc((void**)&s->out_state, sizeof(double) * 4 * nst, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->stage_state, sizeof(double) * 4 * nst, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->stage_up, sizeof(double) * 13 * nst, hf);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->stage_cm, sizeof(double) * n_cm, hf);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->args.queue_head, 16);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ended, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            serve_free(s);
+            return hip_fail(e, "cbc_serve_create allocation");
+        }
+        std::memset(s->ring, 0, sizeof(cbc::ServeItem) * (size_t)cap);
+        std::memset(s->ctl, 0, 64);
+        std::memset(s->done_seq, 0, sizeof(uint32_t) * ns);
+        std::memset(s->out_misc, 0, sizeof(double) * 4 * ns);
+        if (s->out_state) std::memset(s->out_state, 0, sizeof(double) * 4 * nst);
+        std::memset(s->stage_state, 0, sizeof(double) * 4 * nst);
+        std::memset(s->stage_up, 0, sizeof(double) * 13 * nst);
+        std::memset(s->stage_cm, 0, sizeof(double) * n_cm);
+        e = hipMemsetAsync(s->args.queue_head, 0, 16, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            serve_free(s);
+            return hip_fail(e, "cbc_serve_create counter reset");
+        }
+
+        int dev = 0, cus = 0, clk_khz = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, dev);
+        if (e != hipSuccess) {
+            serve_free(s);
+            return hip_fail(e, "device query");
+        }
+        const bool lane = plan->mapping == CBC_MAP_LANE;
+        cbc::KernelAttr at{};
+        e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+        if (e != hipSuccess) {
+            serve_free(s);
+            return hip_fail(e, "hipFuncGetAttributes");
+        }
+        int waves = spec->waves;
+        if (waves == 0) {
+            // enough lanes for every ring item in flight at once, at most one
+            // wave per SIMD slot the register budget allows
+            const int per_wave = lane ? 64 : cbc::group_systems_per_block(plan->exec.ups);
+            const int regs = at.num_regs > 0 ? at.num_regs : 512;
+            const int64_t slots = (int64_t)cus * 4 * (512 / regs > 0 ? 512 / regs : 1);
+            const int64_t want = ((int64_t)std::min<size_t>((size_t)cap, ns) + per_wave - 1) / per_wave;
+            waves = (int)std::max<int64_t>(1, std::min<int64_t>(want, slots));
+        }
+        s->waves = waves;
+
+        // device-side addresses of the host-mapped buffers
+        void *d_ring = nullptr, *d_ctl = nullptr, *d_done = nullptr, *d_misc = nullptr, *d_state = nullptr;
+        void *d_sst = nullptr, *d_sup = nullptr, *d_scm = nullptr;
+        e = hipHostGetDevicePointer(&d_ring, s->ring, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_ctl, s->ctl, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_done, s->done_seq, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_misc, s->out_misc, 0);
+        if (e == hipSuccess && s->out_state) e = hipHostGetDevicePointer(&d_state, s->out_state, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_sst, s->stage_state, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_sup, s->stage_up, 0);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&d_scm, s->stage_cm, 0);
+        if (e != hipSuccess) {
+            serve_free(s);
+            return hip_fail(e, "hipHostGetDevicePointer");
+        }
+        cbc::LaunchArgs& a = s->args;
+        a.kernel_steps = spec->max_attempts;
+        a.refill = spec->refill;
+        a.ring = static_cast<const cbc::ServeItem*>(d_ring);
+        a.ring_tail = static_cast<const uint32_t*>(d_ctl);
+        a.ring_stop = static_cast<const uint32_t*>(d_ctl) + 1;
+        a.ring_mask = (uint32_t)(cap - 1);
+        a.poll_interval = spec->poll_interval;
+        a.idle_ticks = (uint64_t)spec->idle_us * (uint64_t)(clk_khz > 0 ? clk_khz : 100000) / 1000ull;
+        a.done_seq = static_cast<uint32_t*>(d_done);
+        a.out_state = static_cast<double*>(d_state);
+        a.out_misc = static_cast<double*>(d_misc);
+        a.stage_state = static_cast<const double*>(d_sst);
+        a.stage_up = static_cast<const double*>(d_sup);
+        a.stage_cm = static_cast<const double*>(d_scm);
+        a.active_count = nullptr;
+        *out = s;
+        return CBC_OK;
+    } catch (...) {
+        return fail(CBC_ERR_INVALID, "cbc_serve_create: unexpected exception");
+    }
+}
+
+int32_t cbc_serve_get_view(const cbc_serve* s, cbc_serve_view* out)
+{
+    if (!s || !out) return fail(CBC_ERR_INVALID, "cbc_serve_get_view: null argument");
+    out->done_seq = s->done_seq;
+    out->out_state = s->out_state;
+    out->out_misc = s->out_misc;
+    out->stage_state = s->stage_state;
+    out->stage_up = s->stage_up;
+    out->stage_cm = s->stage_cm;
+    out->capacity = s->capacity;
+    out->waves = s->waves;
+    return CBC_OK;
+}
+
+int32_t cbc_serve_submit(cbc_serve* s, const int32_t* sys, const double* pa, int32_t n, double t_end,
+                         const double* h0, const int32_t* opts, uint32_t* seq_out)
+{
+    if (!s) return fail(CBC_ERR_INVALID, "cbc_serve_submit: null handle");
+    if (n < 0 || (n > 0 && (!sys || !pa))) return fail(CBC_ERR_INVALID, "cbc_serve_submit: bad item arrays");
+    if (!(t_end > 0.0) || !(t_end < 1e300)) return fail(CBC_ERR_INVALID, "t_end must be positive and finite");
+    if (n == 0) return CBC_OK;
+    serve_retire(s);
+    const uint32_t used = s->tail - s->retired;
+    if ((int64_t)used + n > (int64_t)s->capacity)
+        return fail(CBC_ERR_BUSY, "serving ring full: %u of %d slots in use, %d requested", used, s->capacity, n);
+    const int32_t ns = s->plan->exec.ns;
+    for (int32_t k = 0; k < n; ++k) {
+        const int32_t q = sys[k];
+        if (q < 0 || q >= ns) return fail(CBC_ERR_INVALID, "system index %d out of [0, %d)", q, ns);
+        const uint32_t done = __atomic_load_n(&s->done_seq[q], __ATOMIC_ACQUIRE);
+        if (done != s->last_seq[q])
+            return fail(CBC_ERR_INVALID, "system %d already has an unfinished item", q);
+        if (opts && (opts[k] & ~CBC_SERVE_RESET) != 0)
+            return fail(CBC_ERR_INVALID, "unknown item option bits 0x%x", (unsigned)opts[k]);
+    }
+    // duplicates within this submission: mark, then check
+    for (int32_t k = 0; k < n; ++k) {
+        const int32_t q = sys[k];
+        if (s->last_seq[q] != __atomic_load_n(&s->done_seq[q], __ATOMIC_ACQUIRE)) {
+            for (int32_t j = 0; j < k; ++j) --s->last_seq[sys[j]];
+            return fail(CBC_ERR_INVALID, "system %d appears twice in one submission", q);
+        }
+        ++s->last_seq[q];
+    }
+    for (int32_t k = 0; k < n; ++k) {
+        const int32_t q = sys[k];
+        cbc::ServeItem& it = s->ring[(s->tail + (uint32_t)k) & (uint32_t)(s->capacity - 1)];
+        it.sys = q;
+        it.seq = s->last_seq[q];
+        it.opts = opts ? opts[k] : 0;
+        it.pad = 0;
+        it.t_end = t_end;
+        it.h0 = h0 ? h0[k] : 0.0;
+        it.pa[0] = pa[k];
+        it.pa[1] = s->nk > 1 ? pa[(size_t)n + k] : 0.0;
+        if (seq_out) seq_out[k] = it.seq;
+    }
+    s->tail += (uint32_t)n;
+    __atomic_store_n(&s->ctl[0], s->tail, __ATOMIC_RELEASE);
+    if (!serve_running(s)) return serve_launch(s);
+    return CBC_OK;
+}
+
+int32_t cbc_serve_poll(cbc_serve* s, int32_t* pending)
+{
+    if (!s) return fail(CBC_ERR_INVALID, "cbc_serve_poll: null handle");
+    serve_retire(s);
+    const uint32_t left = s->tail - s->retired;
+    if (pending) *pending = (int32_t)left;
+    if (left > 0 && !serve_running(s)) {
+        // the kernel has ended (idle timeout) with items it never claimed;
+        // retire again first: items it finished are visible once it has ended
+        serve_retire(s);
+        if (s->tail != s->retired) return serve_launch(s);
+    }
+    return CBC_OK;
+}
+
+int32_t cbc_serve_stats(const cbc_serve* s, int64_t* launches, uint32_t* published, uint32_t* retired)
+{
+    if (!s) return fail(CBC_ERR_INVALID, "cbc_serve_stats: null handle");
+    if (launches) *launches = s->launches;
+    if (published) *published = s->tail;
+    if (retired) *retired = s->retired;
+    return CBC_OK;
+}
+
+int32_t cbc_serve_destroy(cbc_serve* s)
+{
+    if (!s) return CBC_OK;
+    __atomic_store_n(&s->ctl[1], 1u, __ATOMIC_RELEASE);
+    hipError_t e = hipSuccess;
+    if (s->launched) e = hipEventSynchronize(s->ended);
+    serve_free(s);
+    if (e != hipSuccess) return hip_fail(e, "cbc_serve_destroy: waiting for the serving kernel");
     return CBC_OK;
 }
 

@@ -46,7 +46,7 @@ hipError_t group_launch(const LaunchArgs& a, hipStream_t stream)
 hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream)
 {
     const void* fn = group_pick_any(a.ups, a.nk, a.field);
-    if (!fn || !a.queue || !a.queue_head || waves <= 0) return hipErrorInvalidValue;
+    if (!fn || !(a.queue || a.ring) || !a.queue_head || waves <= 0) return hipErrorInvalidValue;
     void* args[] = {const_cast<LaunchArgs*>(&a)};
     return hipLaunchKernel(fn, dim3(waves), dim3(64), args, 0, stream);
 }

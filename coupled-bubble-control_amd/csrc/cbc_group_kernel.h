@@ -15,6 +15,7 @@
 #pragma once
 
 #include "cbc_group.h"
+#include "cbc_serve.h"
 
 namespace cbc {
 
@@ -24,6 +25,7 @@ struct GroupRun {
     double y[4];
     double t, t_end, h, g_old;
     int s, flag, attempts, rejected;
+    uint32_t seq;  // serving: sequence number of the current item
     bool vu;  // lane carries a real bubble of a loaded system
     int didx;
     double dnext, ddt;
@@ -32,10 +34,11 @@ struct GroupRun {
 };
 
 // Per-lane load of system s into group slot L (no barrier).  Returns whether the
-// system is active; a system finished at load gets its flag written.
+// system is active; a system finished at load gets its flag written.  A serving
+// item (`it` != null) starts at tau = 0 as in lane_begin (cbc_lane.hip).
 template <int N, int NK>
 __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, GroupLds<N>& L,
-                                            GroupRun<N, NK>& R)
+                                            GroupRun<N, NK>& R, const ServeItem* it = nullptr)
 {
     const size_t NS = (size_t)a.ns;
     const size_t TILE = (size_t)a.tile;
@@ -44,10 +47,10 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
     R.s = s;
     R.vu = i < N;
     const int iu = R.vu ? i : 0;
-    R.t = a.t[s];
-    R.t_end = a.t_end[s];
-    R.h = a.h[s];
-    R.g_old = a.event[s];
+    R.t = it ? 0.0 : a.t[s];
+    R.t_end = it ? it->t_end : a.t_end[s];
+    R.h = (it && it->h0 > 0.0) ? it->h0 : a.h[s];
+    R.g_old = it ? -1.0 : a.event[s];
     R.flag = kActive;
     R.attempts = 0;
     R.rejected = 0;
@@ -66,9 +69,15 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         return false;
     }
     GroupParams<NK>& P = R.P;
+    // a new episode (serving reset) reads its rows from the host-mapped staging
+    // mirrors and stores them into the device buffers
+    const bool rs = it && (it->opts & kServeReset);
+    const double* src_up = rs ? a.stage_up : a.up;
+    const double* src_cm = rs ? a.stage_cm : a.cm;
+    const double* src_y = rs ? a.stage_state : a.state;
 #pragma unroll
-    for (int j = 0; j < 12; ++j) P.up.u[j] = a.up[j * NST + base + iu];
-    P.rcol = a.up[12 * NST + base + iu];
+    for (int j = 0; j < 12; ++j) P.up.u[j] = ld_src(src_up + j * NST + base + iu, rs);
+    P.rcol = ld_src(src_up + 12 * NST + base + iu, rs);
     P.sp0 = a.sp[s];
     P.sp3 = a.sp[3 * NS + s];
     {
@@ -78,7 +87,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         const double sp4 = a.sp[4 * NS + s];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            const double amps = a.cp[(size_t)k * NS + s];
+            const double amps = it ? it->pa[k] : a.cp[(size_t)k * NS + s];
             const double omega = a.cp[(size_t)(NK + k) * NS + s];
             const double phase = a.cp[(size_t)(2 * NK + k) * NS + s];
             const double wave = a.cp[(size_t)(3 * NK + k) * NS + s];
@@ -97,14 +106,28 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             const size_t off = ((size_t)j * NS + s) * TILE + i;
-            L.mx[0][i][j] = a.cm[off];
-            L.mx[1][i][j] = a.cm[cstride + off];
-            L.mx[2][i][j] = a.cm[2 * cstride + off];
+            L.mx[0][i][j] = ld_src(src_cm + off, rs);
+            L.mx[1][i][j] = ld_src(src_cm + cstride + off, rs);
+            L.mx[2][i][j] = ld_src(src_cm + 2 * cstride + off, rs);
+        }
+        if (rs) {
+            double* up = const_cast<double*>(a.up);
+            double* cm = const_cast<double*>(a.cm);
+#pragma unroll
+            for (int j = 0; j < 12; ++j) up[j * NST + base + i] = P.up.u[j];
+            up[12 * NST + base + i] = P.rcol;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const size_t off = ((size_t)j * NS + s) * TILE + i;
+                cm[off] = L.mx[0][i][j];
+                cm[cstride + off] = L.mx[1][i][j];
+                cm[2 * cstride + off] = L.mx[2][i][j];
+            }
         }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-        R.y[c] = R.vu ? a.state[c * NST + base + i] : (c == 0 ? 1.0 : (c == 1 ? 1e6 * (i + 1) : 0.0));
+        R.y[c] = R.vu ? ld_src(src_y + c * NST + base + i, rs) : (c == 0 ? 1.0 : (c == 1 ? 1e6 * (i + 1) : 0.0));
 
     // dense output (rkck45_coupled_numba_stepper.py:72-112, 333-345)
     R.didx = 0;
@@ -302,6 +325,20 @@ __device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const G
         if (a.rejected_steps) a.rejected_steps[s] += R.rejected;
         if (R.flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
     }
+    if (a.ring) {
+        if (R.vu && a.out_state)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a.out_state[c * NST + base + i] = R.y[c];
+        if (i == 0) {
+            double* m = a.out_misc + (size_t)s * 4;
+            m[0] = R.t;
+            m[1] = R.h;
+            m[2] = (double)R.flag;
+            m[3] = (double)R.attempts;
+        }
+        fence_release_system();
+        if (i == 0) sys_store_u32(a.done_seq + s, R.seq);
+    }
 }
 
 template <int N, int NK, int FIELD>
@@ -338,8 +375,12 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
 
     bool busy = false;
     int64_t used = 0;
-    const bool persistent = a.queue != nullptr;
+    const bool serving = a.ring != nullptr;
+    const bool persistent = a.queue != nullptr || serving;
     bool drained = !persistent;
+    int since_poll = 0x40000000;
+    uint64_t idle_since = 0;
+    int idle_rounds = 0;
     if (!persistent) {
         const int s = blockIdx.x * S + ls;
         if (s < a.ns) busy = group_begin<N, NK>(a, s, i, L, R);
@@ -353,21 +394,42 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
             const uint64_t freemask = __ballot(free_leader);
             const int nfree = __popcll(freemask);
             const bool any_busy = __any(busy);
-            if (nfree > 0 && (nfree >= a.refill || !any_busy)) {
+            bool take = nfree > 0 && (nfree >= a.refill || !any_busy);
+            if (serving) take = take && (!any_busy || since_poll >= a.poll_interval);
+            if (take) {
                 const int leader = __ffsll((unsigned long long)freemask) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
-                base = __shfl(base, leader);
-                if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
                 const uint32_t rank = (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
-                uint32_t idx = free_leader ? base + rank : 0u;
-                idx = __shfl(idx, ls * G);  // group leader's slot to the whole group
-                const int q = (!busy && idx < (uint32_t)a.queue_len) ? a.queue[idx] : -1;
-                const bool gets = q >= 0 && q < a.ns;  // malformed entries are skipped
                 bool fresh = false;
-                if (gets) {
-                    fresh = group_begin<N, NK>(a, q, i, L, R);
-                    used = 0;
+                if (serving) {
+                    since_poll = 0;
+                    uint32_t base = 0, got = 0;
+                    if (lane == leader) got = serve_claim(a, (uint32_t)nfree, &base);
+                    got = __shfl(got, leader);
+                    base = __shfl(base, leader);
+                    if (got > 0) fence_acquire_system();
+                    int slot = (free_leader && rank < got) ? (int)rank : -1;
+                    slot = __shfl(slot, ls * G);  // group leader's slot to the whole group
+                    if (!busy && slot >= 0) {
+                        const ServeItem it = serve_item(a, base + (uint32_t)slot);
+                        if (it.sys >= 0 && it.sys < a.ns) {  // the host validates; never trust the ring
+                            fresh = group_begin<N, NK>(a, it.sys, i, L, R, &it);
+                            R.seq = it.seq;
+                            used = 0;
+                            if (!fresh && i == 0) serve_publish_flag(a, it.sys, it.seq, R.flag);
+                        }
+                    }
+                } else {
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
+                    base = __shfl(base, leader);
+                    if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
+                    uint32_t idx = free_leader ? base + rank : 0u;
+                    idx = __shfl(idx, ls * G);  // group leader's slot to the whole group
+                    const int q = (!busy && idx < (uint32_t)a.queue_len) ? a.queue[idx] : -1;
+                    if (q >= 0 && q < a.ns) {  // malformed entries are skipped
+                        fresh = group_begin<N, NK>(a, q, i, L, R);
+                        used = 0;
+                    }
                 }
                 __syncthreads();
                 const double g0 = group_event<N>(L, i, fresh && R.vu, R.P.rcol, R.y, a.col_factor);
@@ -378,9 +440,15 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
             }
         }
         if (!__any(busy)) {
+            if (serving) {
+                if (serve_idle(a, idle_since, idle_rounds)) break;
+                continue;
+            }
             if (drained) break;
             continue;
         }
+        idle_since = 0;
+        ++since_poll;
         const bool stop = group_attempt<N, NK, FIELD>(a, i, L, R, busy);
         if (busy && (stop || ++used >= a.kernel_steps)) {
             group_finish<N, NK>(a, i, R);

@@ -49,7 +49,46 @@ struct LaunchArgs {
     int32_t queue_len;
     int32_t refill;
     uint32_t* queue_head;
+
+    // serving mode (ring != null, implies persistent): work items are taken from
+    // a host-mapped ring that the host keeps appending to; queue_head counts the
+    // items claimed.  See ServeItem.
+    const struct ServeItem* ring;
+    const uint32_t* ring_tail;   // host-mapped: number of items published
+    const uint32_t* ring_stop;   // host-mapped: nonzero = exit once idle
+    uint32_t ring_mask;          // ring capacity - 1 (capacity a power of two)
+    int32_t poll_interval;       // attempts between ring polls of a wave with free lanes
+    uint64_t idle_ticks;         // wall-clock ticks an idle wave waits for work before exiting
+    uint32_t* done_seq;          // host-mapped (NS): seq of the last item finished per system
+    double* out_state;           // host-mapped (4, NS, TILE) final state copy (may be null)
+    double* out_misc;            // host-mapped (NS, 4): t, h, flag, attempts of the last item
+    // host-mapped staging mirrors (reference layouts) read by items with kServeReset
+    const double* stage_state;   // (4, NS, TILE)
+    const double* stage_up;      // (13, NS, TILE)
+    const double* stage_cm;      // (3, UPS, NS, TILE)
 };
+
+// ServeItem.opts bits
+enum : int32_t {
+    kServeReset = 1,   // new episode: state, unit parameters and coupling rows come from the staging mirrors
+};
+
+// One env step of one system: integrate from the resident state (actual_state,
+// time_step) over tau in [0, t_end] with drive amplitudes pa[0..NK) [Pa]; the
+// stored event value is not consulted (the environment resets it every step).
+// h0 > 0 replaces the resident step size.  With kServeReset the system first
+// takes its state, unit parameters and coupling rows from the staging mirrors
+// and stores them into the device buffers.  48 bytes, written by the host only.
+struct ServeItem {
+    int32_t sys;
+    uint32_t seq;
+    int32_t opts;
+    int32_t pad;
+    double t_end;
+    double h0;
+    double pa[2];
+};
+static_assert(sizeof(ServeItem) == 48, "ServeItem layout");
 
 struct KernelAttr {
     int32_t num_regs;

@@ -10,6 +10,7 @@
 // its lane.
 #include "cbc_device.h"
 #include "cbc_kernels.h"
+#include "cbc_serve.h"
 
 namespace cbc {
 
@@ -168,6 +169,7 @@ struct LaneRun {
     double y[4 * N];
     double t, t_end, h, g_old;
     int s, flag, attempts, rejected;
+    uint32_t seq;             // serving: sequence number of the current item
     int didx;
     double dnext, ddt;
     int wcur;
@@ -176,19 +178,23 @@ struct LaneRun {
 };
 
 // Load system s (rkck45_coupled_numba_stepper.py:179-213, 248-358).  Returns false
-// when the system is already finished at load (its flag is written then).
+// when the system is already finished at load (its flag is written then).  A
+// serving item (`it` != null) starts the system at tau = 0 with the item's end
+// time, amplitudes and optional initial step, ignoring the stored event value
+// (the environment's per-step reset, envs/bubble_1D_pos_control.py:731-749, 815-826).
 template <int N, int NK, int FIELD>
-__device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N, NK>& R)
+__device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N, NK>& R,
+                                           const ServeItem* it = nullptr)
 {
     const size_t NS = (size_t)a.ns;
     const size_t TILE = (size_t)a.tile;
     const size_t NST = NS * TILE;
     const size_t base = (size_t)s * TILE;
     R.s = s;
-    R.t = a.t[s];
-    R.t_end = a.t_end[s];
-    R.h = a.h[s];
-    R.g_old = a.event[s];
+    R.t = it ? 0.0 : a.t[s];
+    R.t_end = it ? it->t_end : a.t_end[s];
+    R.h = (it && it->h0 > 0.0) ? it->h0 : a.h[s];
+    R.g_old = it ? -1.0 : a.event[s];
     R.flag = kActive;
     R.attempts = 0;
     R.rejected = 0;
@@ -201,15 +207,23 @@ __device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N
         }
         if (term) {
             a.flags[s] = flag;
+            R.flag = flag;
             return false;
         }
     }
     LaneSys<N, NK>& P = R.P;
+    // a new episode (serving reset) reads its rows from the host-mapped staging
+    // mirrors and stores them into the device buffers
+    const bool rs = it && (it->opts & kServeReset);
+    const double* src_up = rs ? a.stage_up : a.up;
+    const double* src_cm = rs ? a.stage_cm : a.cm;
+    const double* src_y = rs ? a.stage_state : a.state;
+    const size_t cstride = (size_t)a.ups * NST;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
 #pragma unroll
-        for (int j = 0; j < 12; ++j) P.up[i].u[j] = a.up[j * NST + base + i];
-        P.rcol[i] = a.up[12 * NST + base + i];
+        for (int j = 0; j < 12; ++j) P.up[i].u[j] = ld_src(src_up + j * NST + base + i, rs);
+        P.rcol[i] = ld_src(src_up + 12 * NST + base + i, rs);
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -217,10 +231,26 @@ __device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N
         for (int j = 0; j < N; ++j) {
             // coupling[c][j][s][i] is the effect of unit j on unit i (ckm1d_def.cuh:56-57)
             const size_t off = ((size_t)j * NS + s) * TILE + i;
-            const size_t cstride = (size_t)a.ups * NST;
-            P.m0[i][j] = a.cm[off];
-            P.m1[i][j] = a.cm[cstride + off];
-            P.m2[i][j] = a.cm[2 * cstride + off];
+            P.m0[i][j] = ld_src(src_cm + off, rs);
+            P.m1[i][j] = ld_src(src_cm + cstride + off, rs);
+            P.m2[i][j] = ld_src(src_cm + 2 * cstride + off, rs);
+        }
+    }
+    if (rs) {
+        double* up = const_cast<double*>(a.up);
+        double* cm = const_cast<double*>(a.cm);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+#pragma unroll
+            for (int j = 0; j < 12; ++j) up[j * NST + base + i] = P.up[i].u[j];
+            up[12 * NST + base + i] = P.rcol[i];
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const size_t off = ((size_t)j * NS + s) * TILE + i;
+                cm[off] = P.m0[i][j];
+                cm[cstride + off] = P.m1[i][j];
+                cm[2 * cstride + off] = P.m2[i][j];
+            }
         }
     }
     P.sp0 = a.sp[s];
@@ -232,7 +262,7 @@ __device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N
         const double sp4 = a.sp[4 * NS + s];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            const double amps = a.cp[(size_t)k * NS + s];
+            const double amps = it ? it->pa[k] : a.cp[(size_t)k * NS + s];
             const double omega = a.cp[(size_t)(NK + k) * NS + s];
             const double phase = a.cp[(size_t)(2 * NK + k) * NS + s];
             const double wave = a.cp[(size_t)(3 * NK + k) * NS + s];
@@ -248,7 +278,7 @@ __device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int i = 0; i < N; ++i) R.y[j * N + i] = a.state[j * NST + base + i];
+        for (int i = 0; i < N; ++i) R.y[j * N + i] = ld_src(src_y + j * NST + base + i, rs);
 
     R.g_old = lane_event<N, NK>(P, R.y, a.col_factor);
 
@@ -475,21 +505,42 @@ __device__ __forceinline__ void lane_finish(const LaunchArgs& a, const LaneRun<N
     if (a.total_steps) a.total_steps[s] += R.attempts;
     if (a.rejected_steps) a.rejected_steps[s] += R.rejected;
     if (R.flag == kActive && a.active_count) atomicAdd(a.active_count, 1u);
+    if (a.ring) {
+        if (a.out_state)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < N; ++i) a.out_state[j * NST + base + i] = R.y[j * N + i];
+        double* m = a.out_misc + (size_t)s * 4;
+        m[0] = R.t;
+        m[1] = R.h;
+        m[2] = (double)R.flag;
+        m[3] = (double)R.attempts;
+        fence_release_system();
+        sys_store_u32(a.done_seq + s, R.seq);
+    }
 }
 
 // Plain mode (queue == null): lane = system, at most kernel_steps attempts.
 // Persistent mode: each wave refills its free lanes from the work queue with one
 // atomic per refill (free lanes take consecutive queue slots in lane order) once
 // at least `refill` lanes are free, or all are; the wave exits when the queue is
-// drained and its lanes are idle.  Both modes run the same attempt loop.
+// drained and its lanes are idle.  Serving mode: as persistent mode, but items
+// come from the host-mapped ring (polled every `poll_interval` attempts while
+// some lane is busy), and an idle wave waits for work until the host stops it
+// or `idle_ticks` pass.  All modes run the same attempt loop.
 template <int N, int NK, int FIELD>
 __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
 {
     LaneRun<N, NK> R;
     bool busy = false;
     int64_t used = 0;
-    const bool persistent = a.queue != nullptr;
+    const bool serving = a.ring != nullptr;
+    const bool persistent = a.queue != nullptr || serving;
     bool drained = !persistent;
+    int since_poll = 0x40000000;
+    uint64_t idle_since = 0;
+    int idle_rounds = 0;
     if (!persistent) {
         const int s = blockIdx.x * blockDim.x + threadIdx.x;
         if (s < a.ns) busy = lane_begin<N, NK, FIELD>(a, s, R);
@@ -499,28 +550,55 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
         if (!drained) {
             const uint64_t freemask = __ballot(!busy);
             const int nfree = __popcll(freemask);
-            if (nfree > 0 && (nfree >= a.refill || nfree == 64)) {
+            bool take = nfree > 0 && (nfree >= a.refill || nfree == 64);
+            if (serving) take = take && (nfree == 64 || since_poll >= a.poll_interval);
+            if (take) {
                 const int leader = __ffsll((unsigned long long)freemask) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
-                base = __shfl(base, leader);
-                if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
-                if (!busy) {
-                    const uint32_t idx = base + (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
-                    if (idx < (uint32_t)a.queue_len) {
-                        const int q = a.queue[idx];
-                        if (q >= 0 && q < a.ns) {  // malformed entries are skipped
-                            busy = lane_begin<N, NK, FIELD>(a, q, R);
+                const uint32_t rank = (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
+                if (serving) {
+                    since_poll = 0;
+                    uint32_t base = 0, got = 0;
+                    if (lane == leader) got = serve_claim(a, (uint32_t)nfree, &base);
+                    got = __shfl(got, leader);
+                    base = __shfl(base, leader);
+                    if (got > 0) fence_acquire_system();
+                    if (!busy && rank < got) {
+                        const ServeItem it = serve_item(a, base + rank);
+                        if (it.sys >= 0 && it.sys < a.ns) {  // the host validates; never trust the ring
+                            busy = lane_begin<N, NK, FIELD>(a, it.sys, R, &it);
+                            R.seq = it.seq;
                             used = 0;
+                            if (!busy) serve_publish_flag(a, it.sys, it.seq, R.flag);
+                        }
+                    }
+                } else {
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(a.queue_head, (uint32_t)nfree);
+                    base = __shfl(base, leader);
+                    if (base + (uint32_t)nfree >= (uint32_t)a.queue_len) drained = true;
+                    if (!busy) {
+                        const uint32_t idx = base + rank;
+                        if (idx < (uint32_t)a.queue_len) {
+                            const int q = a.queue[idx];
+                            if (q >= 0 && q < a.ns) {  // malformed entries are skipped
+                                busy = lane_begin<N, NK, FIELD>(a, q, R);
+                                used = 0;
+                            }
                         }
                     }
                 }
             }
         }
         if (!__any(busy)) {
+            if (serving) {
+                if (serve_idle(a, idle_since, idle_rounds)) break;
+                continue;
+            }
             if (drained) break;
             continue;
         }
+        idle_since = 0;
+        ++since_poll;
         if (busy) {
             const bool stop = lane_attempt<N, NK, FIELD>(a, R);
             if (stop || ++used >= a.kernel_steps) {
@@ -587,7 +665,7 @@ hipError_t lane_launch(const LaunchArgs& a, int block, hipStream_t stream)
 hipError_t lane_launch_persistent(const LaunchArgs& a, int block, int waves, hipStream_t stream)
 {
     const void* fn = lane_pick(a.ups, a.nk, a.field);
-    if (!fn || !a.queue || !a.queue_head) return hipErrorInvalidValue;
+    if (!fn || !(a.queue || a.ring) || !a.queue_head) return hipErrorInvalidValue;
     if (block <= 0 || block > 256 || (block % 64) != 0 || waves <= 0) return hipErrorInvalidValue;
     const int wpb = block / 64;
     const int grid = (waves + wpb - 1) / wpb;

@@ -201,6 +201,87 @@ int32_t cbc_launch_queued(cbc_plan *plan, const cbc_buffers *buf, const int32_t 
  * launch left ACTIVE. */
 int32_t cbc_plan_active_count(cbc_plan *plan, void *stream, int32_t *out);
 
+/* ---- serving ------------------------------------------------------------
+ * The vector environment's step (set_action, step_physics, advance_time of
+ * envs/bubble_1D_pos_control.py:731-753, 815-826) as a stream of work items
+ * served by one persistent kernel per plan, with every system's state and last
+ * accepted step size resident in the device buffers between items.  The host
+ * appends items to a ring in coherent pinned host memory; waves with free lanes
+ * (lane groups) claim them as they arrive, so steps of different requests
+ * overlap on the device instead of waiting for one launch per batch.  A
+ * finished item's end state is stored in the device buffers and copied to
+ * host-mapped outputs, then its sequence number is published in done_seq.
+ * Idle waves exit after idle_us without work (or at cbc_serve_destroy); the
+ * next submit or poll relaunches the kernel.  One host thread per handle. */
+typedef struct cbc_serve_spec {
+    int32_t capacity;       /* ring slots: items queued or running at once (power of two, >= 64) */
+    int32_t waves;          /* persistent grid in waves (0 = lanes for min(capacity, ns) systems, at most one wave per SIMD slot) */
+    int32_t refill;         /* free lanes (groups) of a wave that trigger a claim, 1..64 */
+    int32_t poll_interval;  /* attempts between ring polls of a wave that has live systems */
+    int64_t max_attempts;   /* per item; a system reaching it finishes with CBC_ACTIVE */
+    int32_t idle_us;        /* an idle wave exits after this many microseconds without work */
+    int32_t copy_state;     /* nonzero: finished items also copy their state to out_state */
+} cbc_serve_spec;
+
+typedef struct cbc_serve_view {
+    const uint32_t *done_seq;  /* (NS): sequence number of the last finished item per system */
+    const double *out_state;   /* (4, NS, TILE): end state of that item, or NULL */
+    const double *out_misc;    /* (NS, 4): tau reached, step size, status flag, attempts */
+    double *stage_state;       /* (4, NS, TILE) staging rows of new episodes (host writes) */
+    double *stage_up;          /* (13, NS, TILE) */
+    double *stage_cm;          /* (3, UPS, NS, TILE) */
+    int32_t capacity;
+    int32_t waves;
+} cbc_serve_view;
+
+/* item option: the system starts a new episode (reset_envs,
+ * envs/bubble_1D_pos_control.py:583-614, with new radii): its state, unit
+ * parameter and coupling rows are read from the staging mirrors, which the
+ * host filled before submitting, and stored into the device buffers
+ * (actual_state, unit_params, coupling_matrices are then written). */
+#define CBC_SERVE_RESET 1
+
+typedef struct cbc_serve cbc_serve;
+
+/* Creates a serving handle over the plan's device buffers (no counterpart in
+ * the reference, whose environment blocks in model.solve every step,
+ * envs/bubble_1D_pos_control.py:751-753).  Allocates the ring and the outputs
+ * (pinned host memory) and a 16-byte device counter; launches nothing.
+ * stream == NULL: the handle creates its own non-blocking stream.  Work on
+ * another stream that the runtime maps to the same hardware queue waits
+ * until the serving kernel idles out, so the serving loop keeps other device
+ * work (episode resets included) out of the way while serving. */
+int32_t cbc_serve_create(cbc_plan *plan, const cbc_buffers *buf, const cbc_serve_spec *spec, void *stream,
+                         cbc_serve **out);
+
+/* Host pointers of the outputs (valid until cbc_serve_destroy). */
+int32_t cbc_serve_get_view(const cbc_serve *serve, cbc_serve_view *out);
+
+/* Publishes n items (one env step each): system sys[k] integrates tau in
+ * [0, t_end] from its resident state with drive amplitudes pa[k], pa[n + k]
+ * (NK x n, Pa), as set_action + advance_time + step_physics do; h0[k] > 0
+ * replaces the resident step size (the reference's advance_time resets it to
+ * 1e-6), h0[k] <= 0 or h0 == NULL keeps it; opts[k] (may be NULL) holds
+ * CBC_SERVE_* bits.  Each system must have no unfinished item and appear
+ * once.  seq_out[k] (may be NULL) receives the item's sequence number; the
+ * item is finished when done_seq[sys[k]] == seq_out[k].  Returns
+ * CBC_ERR_BUSY, publishing nothing, when fewer than n ring slots are free
+ * (admission).  Launches the serving kernel if none is running. */
+int32_t cbc_serve_submit(cbc_serve *serve, const int32_t *sys, const double *pa, int32_t n, double t_end,
+                         const double *h0, const int32_t *opts, uint32_t *seq_out);
+
+/* Frees the ring slots of finished items and relaunches the kernel if it
+ * ended with items unclaimed; *pending (may be NULL) receives the number of
+ * published items not yet known finished.  Call it while waiting. */
+int32_t cbc_serve_poll(cbc_serve *serve, int32_t *pending);
+
+/* Kernel launches so far, items published, ring slots freed. */
+int32_t cbc_serve_stats(const cbc_serve *serve, int64_t *launches, uint32_t *published, uint32_t *retired);
+
+/* Stops the serving kernel once its live items finish, waits for it and frees
+ * the handle; the device buffers keep every finished item's state. */
+int32_t cbc_serve_destroy(cbc_serve *serve);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,268 @@
+"""Serving mode on the GPU: the persistent ring-fed kernel against the plain kernel
+and the oracle, alone-vs-mix equality of the serving loop, relaunch after the
+kernel idles out, and admission / validation errors.
+
+A serving item runs the same attempt code as a plain launch from the same
+buffers (t = 0, event value reset, amplitudes from the item), so serve and plain
+agree bit for bit; the plain path is pinned to the oracle elsewhere and the
+first test repeats that check for serving directly.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SERVE_TIMEOUT_S = 60.0
+
+
+def _models(n, ns, seed, variant=None):
+    from coupledbubble_control import workloads
+
+    w = workloads.serving_bins(seed=seed, sizes={n: ns})[n]
+    return w, workloads.build_model(w, variant=variant), workloads.build_model(w, variant=variant)
+
+
+def _wait(q, ids, seq):
+    t0 = time.time()
+    while True:
+        q.poll()
+        if q.done(ids, seq).all():
+            return
+        if time.time() - t0 > SERVE_TIMEOUT_S:
+            raise TimeoutError(f"serving items not finished after {SERVE_TIMEOUT_S} s")
+        time.sleep(2e-4)
+
+
+def _plain_step(m, ids, pa_pa, periods):
+    """One plain launch of a step for systems ``ids`` (others parked)."""
+    import torch
+
+    from coupledbubble_control.backends.hip import rkck45_coupled_hip_solver as hs
+
+    m._ensure_buffers()
+    if m.runtime.solver is None:
+        m.runtime.solver = m._make_solver(m.runtime.backend, m.runtime.variant)
+    db = m.runtime.device_buffers
+    dev = db.state.actual_state.device
+    ti = torch.from_numpy(np.asarray(ids, dtype=np.int64)).to(dev)
+    db.state.time_end.copy_(db.state.actual_time)
+    for k in range(m._model_spec.nk):
+        db.params.control_params[k].index_copy_(0, ti, torch.from_numpy(np.ascontiguousarray(pa_pa[k])).to(dev))
+    db.state.actual_time.index_fill_(0, ti, 0.0)
+    db.state.time_end.index_fill_(0, ti, float(periods))
+    db.status.actual_event.index_fill_(0, ti, -1.0)
+    db.status.status_flags.index_fill_(0, ti, 0)
+    hs.launch(m.runtime.kernel, db, 1 << 20)
+    torch.cuda.synchronize()
+
+
+def _rows(m, ids):
+    db = m.runtime.device_buffers
+    n = m._execution_spec.ups
+    ids = np.asarray(ids)
+    return (db.state.actual_state.cpu().numpy()[:, ids, :n], db.state.time_step.cpu().numpy()[ids],
+            db.state.actual_time.cpu().numpy()[ids], db.status.status_flags.cpu().numpy()[ids])
+
+
+@pytest.mark.parametrize("n,ns,variant", [(2, 256, None), (4, 128, None), (2, 128, "group")])
+def test_serve_equals_plain_bitwise(n, ns, variant):
+    from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
+
+    print(f"\n[serve==plain] N={n} ns={ns} variant={variant}", flush=True)
+    _, ms, mp = _models(n, ns, seed=11, variant=variant)
+    rng = np.random.default_rng(5)
+    with ServeQueue(ms, ServeOptions(capacity=1024, max_attempts=200000, idle_us=2000)) as q:
+        steps = [np.arange(ns), rng.choice(ns, size=ns // 3, replace=False)]
+        for k, ids in enumerate(steps):
+            pa = rng.uniform(0.0, 1.5, size=(2, ids.size)) * 1e5
+            seq = q.submit(ids, pa, 5.0)
+            assert seq is not None
+            _wait(q, ids, seq)
+            _plain_step(mp, ids, pa, 5.0)
+            ys, hs_, ts, fs = _rows(ms, ids)
+            yp, hp, tp, fp = _rows(mp, ids)
+            np.testing.assert_array_equal(fs, fp)
+            np.testing.assert_array_equal(ts, tp)
+            np.testing.assert_array_equal(hs_, hp)
+            np.testing.assert_array_equal(ys, yp)
+            st, misc = q.results(ids)
+            np.testing.assert_array_equal(st, ys)
+            np.testing.assert_array_equal(misc[:, 2].astype(np.int32), fs)
+            np.testing.assert_array_equal(misc[:, 0], ts)
+            print(f"  step {k}: {ids.size} systems, flags {np.unique(fs, return_counts=True)}", flush=True)
+
+
+@pytest.mark.parametrize("n,ns", [(2, 128), (4, 64)])
+def test_serve_reset_item_equals_plain_reset(n, ns):
+    """A new episode staged in host memory and started by a reset item gives the device
+    rows and end states of reset_systems followed by a plain launch."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
+    from coupledbubble_control.serving import c0_scene
+
+    print(f"\n[serve reset] N={n} ns={ns}", flush=True)
+    _, ms, mp = _models(n, ns, seed=16)
+    lam = workloads.wavelength(workloads.water_25c(), 25.0)
+    rng = np.random.default_rng(8)
+    ids = np.arange(ns)
+    with ServeQueue(ms, ServeOptions(capacity=256, max_attempts=200000, idle_us=2000)) as q:
+        pa = rng.uniform(0.0, 1.5, size=(2, ns)) * 1e5
+        _wait(q, ids, q.submit(ids, pa, 2.0))
+        _plain_step(mp, ids, pa, 2.0)
+        rid = np.sort(rng.choice(ns, size=ns // 4, replace=False))
+        R0, X0 = c0_scene(n, rid.size, np.random.default_rng(1), lam)
+        q.stage(rid, *ms.scene_rows(rid, R0, X0))
+        mp.reset_systems(rid, R0, X0, h0=1e-6)
+        torch.cuda.synchronize()
+        reset = np.isin(ids, rid)
+        pa2 = rng.uniform(0.0, 1.5, size=(2, ns)) * 1e5
+        _wait(q, ids, q.submit(ids, pa2, 2.0, np.where(reset, 1e-6, 0.0), reset))
+        _plain_step(mp, ids, pa2, 2.0)
+    for a, b in zip(_rows(ms, ids), _rows(mp, ids)):
+        np.testing.assert_array_equal(a, b)
+    for name in ("unit_params", "coupling_matrices"):
+        np.testing.assert_array_equal(getattr(ms.runtime.device_buffers.params, name).cpu().numpy(),
+                                      getattr(mp.runtime.device_buffers.params, name).cpu().numpy())
+
+
+def test_serve_vs_oracle():
+    """One served step of 5 periods against the C oracle from the same state."""
+    import test_gpu_plain as P
+
+    from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
+    from oracle import ckm1d_oracle as O
+
+    print("\n[serve vs oracle] N=2 ns=128", flush=True)
+    _, m, _ = _models(2, 128, seed=12)
+    ref = O.host_buffers_dict(m)
+    spec = O.spec_from_model(m)
+    rng = np.random.default_rng(6)
+    ids = np.arange(128)
+    pa = rng.uniform(0.0, 1.5, size=(2, 128)) * 1e5
+    ref["control_params"][0:2] = pa
+    ref["actual_time"][:] = 0.0
+    ref["time_end"][:] = 5.0
+    ref["actual_event"][:] = -1.0
+    O.solve(spec, ref, kernel_steps=1 << 20, max_steps=1 << 20)
+    with ServeQueue(m, ServeOptions(capacity=256, max_attempts=200000, idle_us=2000)) as q:
+        seq = q.submit(ids, pa, 5.0)
+        _wait(q, ids, seq)
+    m.buffer_ops.sync_to_host(m.runtime.device_buffers.state, m.runtime.host_buffers.state)
+    m.buffer_ops.sync_to_host(m.runtime.device_buffers.status, m.runtime.host_buffers.status)
+    assert np.array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+    err = P._assert_states(m, ref, 2)
+    print(f"  max rel err {err:.2e}", flush=True)
+
+
+def _engine_run(bins_w, reqs, mode):
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends.hip.serve import ServeOptions
+    from coupledbubble_control.serving import ServingEngine
+
+    bins = {n: workloads.build_model(w) for n, w in bins_w.items()}
+    eng = ServingEngine(bins, periods=5.0, mode=mode,
+                        serve_opts=ServeOptions(max_attempts=200000, idle_us=5000))
+    try:
+        stats = eng.run(reqs, timeout_s=120.0)
+    finally:
+        eng.close()
+    out = {}
+    for r in eng.completed:
+        y, h, t, f = _rows(bins[r.n], r.env_ids)
+        out[r.rid] = (y, h, t, f, r.flags.copy())
+    return stats, out
+
+
+def _fresh_requests(venvs, seed):
+    from coupledbubble_control.serving import Request
+
+    rng = np.random.default_rng(seed)
+    return [Request(i, n, i, ids, rng.uniform(0.0, 1.5, size=(2, ids.size)), 0.0) for i, (n, ids) in enumerate(venvs)]
+
+
+def test_serving_alone_vs_mix():
+    """Every request's results are the same served alone, in the mix, and by the plain
+    one-launch-per-batch path."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.serving import vector_envs
+
+    print("\n[alone vs mix] bins N=2 x 384, N=4 x 192", flush=True)
+    bins_w = workloads.serving_bins(seed=13, sizes={2: 384, 4: 192})
+    venvs = vector_envs({2: 384, 4: 192}, np.random.default_rng(3), lo=64, hi=160)
+    reqs = _fresh_requests(venvs, 21)
+    st_mix, mix = _engine_run(bins_w, reqs, "continuous")
+    print(f"  mix: {st_mix}", flush=True)
+    assert st_mix["requests"] == len(reqs)
+    _, plain = _engine_run(bins_w, _fresh_requests(venvs, 21), "plain")
+    for rid in mix:
+        for a, b in zip(mix[rid], plain[rid]):
+            np.testing.assert_array_equal(a, b)
+    for pick in (0, len(reqs) - 1):
+        one = [r for r in _fresh_requests(venvs, 21) if r.rid == pick]
+        _, alone = _engine_run(bins_w, one, "continuous")
+        for a, b in zip(mix[pick], alone[pick]):
+            np.testing.assert_array_equal(a, b)
+        print(f"  request {pick} ({reqs[pick].size} envs, N={reqs[pick].n}) alone == mix", flush=True)
+
+
+def test_serve_relaunch_after_idle_and_admission():
+    from coupledbubble_control.backends.hip import _abi
+    from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
+
+    print("\n[relaunch/admission] N=2 ns=256", flush=True)
+    _, m, _ = _models(2, 256, seed=14)
+    with ServeQueue(m, ServeOptions(capacity=128, max_attempts=200000, idle_us=500)) as q:
+        ids = np.arange(64)
+        pa = np.full((2, 64), 0.5e5)
+        seq = q.submit(ids, pa, 5.0)   # five periods: thousands of attempts, so still running below
+        # a system with an unfinished item is refused
+        with pytest.raises(_abi.CBCError):
+            q.submit(np.arange(0, 1), np.full((2, 1), 0.5e5), 1.0)
+        # ring full: 64 of 128 slots are in use until these items finish
+        assert q.submit(np.arange(64, 129), np.full((2, 65), 0.5e5), 1.0) is None
+        _wait(q, ids, seq)
+        time.sleep(0.05)   # > idle_us: the kernel idles out
+        q.poll()
+        assert q.stats()[0] == 1
+        with pytest.raises(_abi.CBCError):
+            q.submit(np.array([70, 70]), np.full((2, 2), 0.5e5), 1.0)   # duplicate system
+        ids2 = np.arange(64, 128)
+        seq2 = q.submit(ids2, pa, 1.0)
+        assert seq2 is not None
+        _wait(q, ids2, seq2)
+        launches, published, _ = q.stats()
+        assert launches == 2 and published == 128
+        flags = q.results(ids2)[1][:, 2]
+        assert set(np.unique(flags)).issubset({1.0, 2.0, -2.0})  # SUCCESS, collision, h below resolution
+
+
+def test_serving_loop_with_resets_runs():
+    """A short Poisson run of the serving loop with episode resets completes every request."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends.hip.serve import ServeOptions
+    from coupledbubble_control.serving import ServingEngine, c0_scene, make_requests, vector_envs
+
+    print("\n[serving loop] N=2 x 1024, N=4 x 512, 0.5 s of arrivals", flush=True)
+    sizes = {2: 1024, 4: 512}
+    bins_w = workloads.serving_bins(seed=15, sizes=sizes)
+    venvs = vector_envs(sizes, np.random.default_rng(4))
+    reqs = make_requests(venvs, rate_env_steps=20000.0, duration=0.5, seed=7)
+    lam = workloads.wavelength(bins_w[2].mat, 25.0)
+    bins = {n: workloads.build_model(w) for n, w in bins_w.items()}
+    eng = ServingEngine(bins, periods=5.0, mode="continuous", reset_prob=0.05,
+                        reset_scene=lambda n, k, rng: c0_scene(n, k, rng, lam),
+                        serve_opts=ServeOptions(max_attempts=200000, idle_us=5000))
+    try:
+        st = eng.run(reqs, timeout_s=120.0)
+    finally:
+        eng.close()
+    print(f"  {st}", flush=True)
+    assert st["requests"] == len(reqs)
+    assert st["resets"] > 0
+    for m in bins.values():
+        y = m.runtime.device_buffers.state.actual_state.cpu().numpy()
+        assert np.isfinite(y).all()
