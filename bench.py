@@ -12,7 +12,11 @@ bubbles with implicit radial/translational coupling):
   ``--steps`` requests are timed.  The same requests are then served by the
   plain path (one kernel launch per batch, per bubble count), and both paths
   are also driven closed-loop (each vector env resubmits as soon as its step
-  completes) for ``--saturation-s`` seconds to measure capacity.
+  completes) for ``--saturation-s`` seconds to measure capacity.  "Plain" is
+  static batching: everything queued when the previous batch has finished is
+  one batch (one launch per bubble count), and the next batch starts when all
+  of its launches have ended; ``plain_per_bin`` also reports one such batch
+  loop per bubble count.
   value = integrated periods / wall seconds of the continuous path under the
   Poisson load; latency = request arrival to completion.
 * ``param_study`` -- 65 536 systems of N=2 on a 256 x 256 (PA1, PA2) grid over
@@ -147,16 +151,16 @@ def bench_serving(args):
     venvs = vector_envs(sizes, np.random.default_rng(3))
     lam = W.wavelength(W.water_25c(), 25.0)
 
-    def engine(mode):
+    def engine(mode, per_bin=False):
         bins_w = W.serving_bins(seed=3, sizes=sizes, periods=K)
         bins = {n: W.build_model(w) for n, w in bins_w.items()}
         eng = ServingEngine(bins, periods=K, mode=mode, reset_prob=0.001,
                             reset_scene=lambda n, k, rng: c0_scene(n, k, rng, lam),
-                            serve_opts=ServeOptions(max_attempts=1 << 20), seed=3)
+                            serve_opts=ServeOptions(max_attempts=1 << 20), seed=3, plain_per_bin=per_bin)
         return eng, bins
 
-    def run(mode, reqs=None, closed=None):
-        eng, bins = engine(mode)
+    def run(mode, reqs=None, closed=None, per_bin=False):
+        eng, bins = engine(mode, per_bin)
         a0 = attempts_of(bins.values())
         try:
             if closed is not None:
@@ -179,9 +183,13 @@ def bench_serving(args):
     log(f"continuous: {steps} requests at {args.rate:g} env-steps/s")
     cont, info = run("continuous", make_requests(venvs, args.rate, count=steps, seed=3))
     log(f"continuous: {cont['periods_per_s']:.4g} periods/s p50 {cont['p50_ms']:.2f} ms p99 {cont['p99_ms']:.2f} ms")
-    log("plain: same requests")
+    log("plain (static batches): same requests")
     plain, _ = run("plain", make_requests(venvs, args.rate, count=steps, seed=3))
     log(f"plain: {plain['periods_per_s']:.4g} periods/s p50 {plain['p50_ms']:.2f} ms p99 {plain['p99_ms']:.2f} ms")
+    log("plain, one batch loop per bubble count: same requests")
+    plain_bin, _ = run("plain", make_requests(venvs, args.rate, count=steps, seed=3), per_bin=True)
+    log(f"plain per bin: {plain_bin['periods_per_s']:.4g} periods/s p50 {plain_bin['p50_ms']:.2f} ms "
+        f"p99 {plain_bin['p99_ms']:.2f} ms")
     sat = None
     if args.saturation_s > 0:
         log(f"closed loop {args.saturation_s:g} s: continuous")
@@ -218,6 +226,7 @@ def bench_serving(args):
         "env_steps_per_s": cont["env_steps_per_s"],
         "continuous": {k: cont[k] for k in keys},
         "plain": {k: plain[k] for k in keys},
+        "plain_per_bin": {k: plain_bin[k] for k in keys},
         "saturation": sat,
         "roofline": roofline(cont["flops"], cont["wall_s"], window="timed region (persistent serving kernels)",
                              flops_per_attempt={str(n): flops_per_attempt(n) for n in sizes},

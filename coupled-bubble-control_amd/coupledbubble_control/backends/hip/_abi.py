@@ -45,6 +45,9 @@ CBC_LIN_DIRECT = 0
 
 CBC_SERVE_RESET = 1
 
+MATH_CODES = {"sincos": 0, "exp": 1, "log": 2, "pow": 3, "rcp": 4, "sincos_kern": 5, "sincos_tiny": 6,
+              "rotate": 7, "pow1p": 8}
+
 FIELD_CODES = {"CONST": CBC_FIELD_CONST, "SW_N": CBC_FIELD_SW_N, "SW_A": CBC_FIELD_SW_A}
 
 _i32 = ctypes.c_int32
@@ -120,6 +123,7 @@ SIGNATURES = {
     "cbc_serve_stats": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_uint32),
                                ctypes.POINTER(ctypes.c_uint32)]),
     "cbc_serve_destroy": (_i32, [_vp]),
+    "cbc_selftest_math": (_i32, [_i32, _vp, _vp, _vp, _i32, _vp]),
 }
 
 

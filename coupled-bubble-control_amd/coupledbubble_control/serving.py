@@ -20,10 +20,12 @@ amplitudes for its envs and K acoustic periods to integrate.
   (``backends.hip.serve.ServeQueue``); admitted requests are appended to its
   work ring at once and waves with free lanes pick them up while other
   requests are still integrating.
-* ``mode="plain"``: one launch per batch, per bubble count: when a bin's
-  previous launch has finished, every admitted request of that bin is written
-  into the device buffers and one kernel launch integrates them; all complete
-  together when the launch ends (the reference's blocking ``env.step``).
+* ``mode="plain"``: static batching, one launch per batch: when the previous
+  batch has finished, every admitted request is written into the device
+  buffers and one kernel launch per bubble count integrates them; the batch
+  completes when its last launch ends (the reference's blocking ``env.step``
+  over everything queued).  ``plain_per_bin=True`` runs one such batch loop per
+  bubble count, so the bins' batches overlap each other.
 * After a request completes, its envs that did not end with SUCCESS (as in
   ``check_termination``, :771-789) and, with probability ``reset_prob`` per
   step, any of its envs start new episodes with fresh radii and positions.
@@ -136,7 +138,7 @@ class ServingEngine:
     def __init__(self, bins: Dict[int, object], periods: float = 5.0, mode: str = "continuous",
                  budget_bytes: int = 64 << 20, serve_opts=None, reset_prob: float = 0.0,
                  reset_scene: Optional[Callable] = None, seed: int = 0, keep_step: bool = True,
-                 plain_kernel_steps: int = 1 << 20):
+                 plain_kernel_steps: int = 1 << 20, plain_per_bin: bool = False):
         import torch
 
         from .backends.hip import rkck45_coupled_hip_solver as hs
@@ -154,6 +156,7 @@ class ServingEngine:
         self.reset_scene = reset_scene
         self.rng = np.random.default_rng(seed)
         self.plain_kernel_steps = int(plain_kernel_steps)
+        self.plain_per_bin = bool(plain_per_bin)
         per_sys = max(inflight_bytes(m._execution_spec.ups, m._execution_spec.tile) for m in self.bins.values())
         self.admission = Admission(max(1, int(budget_bytes) // per_sys))
         self.queues: Dict[int, object] = {}
@@ -301,28 +304,26 @@ class ServingEngine:
             r.start = now
         self.batches[n] = (ev, reqs, ti)
 
-    def _collect_plain(self, now: float, block: bool) -> int:
+    def _collect_plain(self, now: float) -> int:
+        """Complete finished launches: per bin, or the whole batch once all its
+        launches have ended."""
+        ready = [n for n, (ev, _, _) in self.batches.items() if ev.query()]
+        if not self.plain_per_bin and len(ready) < len(self.batches):
+            return 0
         n_done = 0
-        for n in list(self.batches):
-            ev, reqs, ti = self.batches[n]
-            if not ev.query():
-                if not block:
-                    continue
-                ev.synchronize()
-                now = self._now()
+        for n in ready:
+            ev, reqs, ti = self.batches.pop(n)
             db = self.bins[n].runtime.device_buffers
             with self.torch.cuda.stream(self.streams[n]):
                 flags = db.status.status_flags.index_select(0, ti).cpu().numpy()
                 t_now = db.state.actual_time.index_select(0, ti)
                 t = t_now.cpu().numpy()
                 db.state.time_end.index_copy_(0, ti, t_now)   # park the batch again
-            del self.batches[n]
             off = 0
             for r in reqs:
                 self._finish(r, flags[off:off + r.size], t[off:off + r.size], now)
                 off += r.size
                 n_done += 1
-            block = False
         return n_done
 
     # -- main loop -------------------------------------------------------------------
@@ -353,14 +354,22 @@ class ServingEngine:
                 for q in self.queues.values():
                     q.poll()
             else:
-                done = self._collect_plain(now, block=False)
+                done = self._collect_plain(now)
                 adm = []
-                for n in self.bins:
-                    if n not in self.batches:
-                        a = self.admission.admit(only_bin=n)
-                        if a:
-                            self._launch_batch(n, a, now)
-                            adm += a
+                if self.plain_per_bin:
+                    for n in self.bins:
+                        if n not in self.batches:
+                            a = self.admission.admit(only_bin=n)
+                            if a:
+                                self._launch_batch(n, a, now)
+                                adm += a
+                elif not self.batches:
+                    adm = self.admission.admit()
+                    by_bin: Dict[int, List[Request]] = {}
+                    for r in adm:
+                        by_bin.setdefault(r.n, []).append(r)
+                    for n, rs in by_bin.items():
+                        self._launch_batch(n, rs, now)
             if not (moved or done or adm):
                 time.sleep(2e-5)
             if progress and now - last_report > 5.0:
@@ -375,7 +384,8 @@ class ServingEngine:
         env_steps = sum(r.size for r in self.completed)
         periods = sum(r.periods for r in self.completed)
         failed = sum(int((r.flags != 1).sum()) for r in self.completed)
-        out = {"mode": self.mode, "wall_s": wall, "requests": len(self.completed), "env_steps": env_steps,
+        mode = self.mode + ("_per_bin" if self.mode == "plain" and self.plain_per_bin else "")
+        out = {"mode": mode, "wall_s": wall, "requests": len(self.completed), "env_steps": env_steps,
                "periods": periods, "periods_per_s": periods / wall, "env_steps_per_s": env_steps / wall,
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "failed_env_steps": failed,

@@ -16,6 +16,7 @@
 
 #include "../../include/cbc.h"
 #include "cbc_kernels.h"
+#include "cbc_math.h"
 
 namespace {
 
@@ -56,6 +57,36 @@ hipError_t zero_async(uint32_t* p, int n, hipStream_t st)
 {
     void* args[] = {&p, &n};
     return hipLaunchKernel(reinterpret_cast<const void*>(&zero_words), dim3((n + 63) / 64), dim3(64), args, 0, st);
+}
+
+// Device evaluation of the elementary functions for cbc_selftest_math.
+__global__ void selftest_math(int fn, const double* x, const double* y, double* out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i];
+    const double b = y ? y[i] : 0.0;
+    double u = 0.0, v = 0.0;
+    switch (fn) {
+    case CBC_MATH_SINCOS: cbc::sincos_fast(a, &u, &v); break;
+    case CBC_MATH_EXP: u = cbc::exp_fast(a); break;
+    case CBC_MATH_LOG: u = cbc::log_fast(a); break;
+    case CBC_MATH_POW: u = cbc::pow_pos(a, b); break;
+    case CBC_MATH_RCP: u = cbc::rcp_fast(a); break;
+    case CBC_MATH_SINCOS_KERN: cbc::sincos_kern(a, &u, &v); break;
+    case CBC_MATH_SINCOS_TINY: cbc::sincos_tiny(a, &u, &v); break;
+    case CBC_MATH_ROTATE: {
+        double sa, ca, sb, cm1;
+        cbc::sincos_fast(a, &sa, &ca);
+        cbc::sincos_tiny(b, &sb, &cm1);
+        cbc::rotate(sa, ca, sb, cm1, &u, &v);
+        break;
+    }
+    case CBC_MATH_POW1P: u = cbc::pow1p_small(a, b); break;
+    default: break;
+    }
+    out[2 * (size_t)i] = u;
+    out[2 * (size_t)i + 1] = v;
 }
 
 }  // namespace
@@ -357,6 +388,21 @@ int32_t cbc_launch_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t*
     if (rc != CBC_OK) return rc;
     hipError_t e = queued_launch(plan, a, waves, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "persistent kernel launch");
+    return CBC_OK;
+}
+
+int32_t cbc_selftest_math(int32_t fn, const double* x, const double* y, double* out, int32_t n, void* stream)
+{
+    if (fn < CBC_MATH_SINCOS || fn > CBC_MATH_POW1P) return fail(CBC_ERR_INVALID, "unknown function code %d", fn);
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(CBC_ERR_INVALID, "cbc_selftest_math: bad arrays");
+    if ((fn == CBC_MATH_POW || fn == CBC_MATH_ROTATE || fn == CBC_MATH_POW1P) && n > 0 && !y)
+        return fail(CBC_ERR_INVALID, "function %d needs the second argument array", fn);
+    if (n == 0) return CBC_OK;
+    int f = fn;
+    void* args[] = {&f, &x, &y, &out, &n};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&selftest_math), dim3((n + 255) / 256), dim3(256),
+                                   args, 0, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "selftest kernel launch");
     return CBC_OK;
 }
 

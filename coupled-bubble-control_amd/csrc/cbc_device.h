@@ -51,6 +51,20 @@ struct CK {
                             e5 = c5 - c5p, e6 = c6 - c6p;
 };
 
+// The tableau by stage index q = 0..5 for the rolled stage loop: kCkA[q] stage
+// time fraction, kCkB[r][q] weight of k_q in the input of stage r (0 unless
+// q < r), kCkC / kCkE the 5th-order and embedded-error weights.
+__device__ constexpr double kCkA[6] = {0.0, CK::a2, CK::a3, CK::a4, CK::a5, CK::a6};
+__device__ constexpr double kCkB[6][6] = {
+    {0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {CK::b21, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {CK::b31, CK::b32, 0.0, 0.0, 0.0, 0.0},
+    {CK::b41, CK::b42, CK::b43, 0.0, 0.0, 0.0},
+    {CK::b51, CK::b52, CK::b53, CK::b54, 0.0, 0.0},
+    {CK::b61, CK::b62, CK::b63, CK::b64, CK::b65, 0.0}};
+__device__ constexpr double kCkC[6] = {CK::c1, CK::c2, CK::c3, CK::c4, CK::c5, CK::c6};
+__device__ constexpr double kCkE[6] = {CK::e1, CK::e2, CK::e3, CK::e4, CK::e5, CK::e6};
+
 __device__ __forceinline__ double sgn(double v) { return (double)((v > 0.0) - (v < 0.0)); }
 
 // Per-system constants of the acoustic drive, folded once per launch from the
@@ -68,10 +82,12 @@ struct Drive {
 };
 
 // Pressure, its time derivative, its gradient and the particle velocity at
-// (t, x) given the time harmonics sin/cos(th_k); ckm1d_def.py:40-155.
+// (t, x) given the time harmonics st/ct[k] = sin/cos(wt_k t + ph_k) and the
+// space harmonics sx/cx[k] = sin/cos(wx_k x + ph_k); ckm1d_def.py:40-155.
 template <int NK, int FIELD>
-__device__ __forceinline__ void field_at(const Drive<NK>& d, const double (&st)[NK], const double (&ct)[NK],
-                                         double x, double& p, double& pt, double& px, double& ux)
+__device__ __forceinline__ void field_from(const Drive<NK>& d, const double (&st)[NK], const double (&ct)[NK],
+                                           const double (&sx)[NK], const double (&cx)[NK],
+                                           double& p, double& pt, double& px, double& ux)
 {
     p = 0.0; pt = 0.0; px = 0.0; ux = 0.0;
 #pragma unroll
@@ -79,22 +95,100 @@ __device__ __forceinline__ void field_at(const Drive<NK>& d, const double (&st)[
         if constexpr (FIELD == kFieldConst) {
             p += d.A[k] * st[k];
             pt += d.Aw[k] * ct[k];
+        } else if constexpr (FIELD == kFieldSwN) {
+            p += d.A[k] * sx[k] * st[k];
+            pt += d.Aw[k] * sx[k] * ct[k];
+            px += d.Ak[k] * cx[k] * st[k];
+            ux += d.Au[k] * cx[k] * ct[k];
         } else {
-            double sx, cx;
-            sincos_fast(d.wx[k] * x + d.ph[k], &sx, &cx);
-            if constexpr (FIELD == kFieldSwN) {
-                p += d.A[k] * sx * st[k];
-                pt += d.Aw[k] * sx * ct[k];
-                px += d.Ak[k] * cx * st[k];
-                ux += d.Au[k] * cx * ct[k];
-            } else {
-                p += d.A[k] * cx * st[k];
-                pt += d.Aw[k] * cx * ct[k];
-                px -= d.Ak[k] * sx * st[k];
-                ux -= d.Au[k] * sx * ct[k];
-            }
+            p += d.A[k] * cx[k] * st[k];
+            pt += d.Aw[k] * cx[k] * ct[k];
+            px -= d.Ak[k] * sx[k] * st[k];
+            ux -= d.Au[k] * sx[k] * ct[k];
         }
     }
+}
+
+// Stage values reused within one attempt.  The first stage (t, x, x0 = R/R0)
+// is evaluated in full; later stages (tq, xq, x0q) start from it by the
+// addition theorems and the (1+e)^y series when the lane's own arguments are
+// inside their bounds, else in full.  The choice is made per lane, never per
+// wave, so a system's arithmetic does not depend on the systems sharing its
+// wave.  Lanes with vote = false (no live bubble) always take the series
+// path, so they never make the wave run both.
+
+// time harmonics sin/cos(wt_k t + ph_k)
+template <int NK>
+__device__ __forceinline__ void time_base(const Drive<NK>& d, double t, double (&st)[NK], double (&ct)[NK])
+{
+#pragma unroll
+    for (int k = 0; k < NK; ++k) sincos_fast(d.wt[k] * t + d.ph[k], &st[k], &ct[k]);
+}
+
+template <int NK>
+__device__ __forceinline__ void time_stage(const Drive<NK>& d, const double (&sb0)[NK], const double (&cb0)[NK],
+                                           double t, double tq, bool vote, double (&st)[NK], double (&ct)[NK])
+{
+    const double dt = tq - t;
+    double b[NK];
+    bool far = false;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        b[k] = d.wt[k] * dt;
+        far |= !(fabs(b[k]) <= kPi4);
+    }
+    if (vote && far) {
+        time_base<NK>(d, tq, st, ct);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            double sb, cm1;
+            sincos_kern(b[k], &sb, &cm1);
+            rotate(sb0[k], cb0[k], sb, cm1, &st[k], &ct[k]);
+        }
+    }
+}
+
+// space harmonics sin/cos(wx_k x + ph_k) of one bubble
+template <int NK>
+__device__ __forceinline__ void space_base(const Drive<NK>& d, double x, double (&sx)[NK], double (&cx)[NK])
+{
+#pragma unroll
+    for (int k = 0; k < NK; ++k) sincos_fast(d.wx[k] * x + d.ph[k], &sx[k], &cx[k]);
+}
+
+template <int NK>
+__device__ __forceinline__ bool space_far(const Drive<NK>& d, double xb, double xq)
+{
+    const double dx = xq - xb;
+    bool far = false;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) far |= !(fabs(d.wx[k] * dx) <= kTinyAngle);
+    return far;
+}
+
+template <int NK>
+__device__ __forceinline__ void space_shift(const Drive<NK>& d, const double (&sb0)[NK], const double (&cb0)[NK],
+                                            double xb, double xq, double (&sx)[NK], double (&cx)[NK])
+{
+    const double dx = xq - xb;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        double sb, cm1;
+        sincos_tiny(d.wx[k] * dx, &sb, &cm1);
+        rotate(sb0[k], cb0[k], sb, cm1, &sx[k], &cx[k]);
+    }
+}
+
+// polytropic factor (R/R0)^(-sp0) from its first-stage value pf0 at x0b (rxb = 1/x0b)
+__device__ __forceinline__ bool pf_far(double sp0, double x0b, double rxb, double x0q)
+{
+    return !(fabs((x0q - x0b) * rxb) <= kSmallRatio) || !(fabs(sp0) <= kSmallExp);
+}
+
+__device__ __forceinline__ double pf_shift(double sp0, double pf0, double x0b, double rxb, double x0q)
+{
+    return pf0 * pow1p_small((x0q - x0b) * rxb, -sp0);
 }
 
 // Per-bubble constants: up[0..11] of models/ckm1d.py:153-167.
@@ -110,13 +204,12 @@ struct BubbleOut {
     double H0, H1, H2, H3, H4, H5, H6;
 };
 
-__device__ __forceinline__ void bubble_rhs(const Unit& up, double sp0, double sp3,
-                                           double x0, double x2, double x3,
-                                           double pa, double pat, double pax, double uax,
+__device__ __forceinline__ void bubble_rhs(const Unit& up, double sp3, double x0, double x2, double x3,
+                                           double pa, double pat, double pax, double uax, double p,
                                            BubbleOut& o)
 {
+    // p = (1/x0)^sp0, the polytropic factor (Harm::pf)
     const double rx0 = rcp_fast(x0);
-    const double p = pow_pos(x0, -sp0);  // (1/x0)^sp0
     const double N = (up.u[0] + up.u[1] * x2) * p
                    - up.u[2] * (1 + up.u[7] * x2)
                    - up.u[3] * rx0

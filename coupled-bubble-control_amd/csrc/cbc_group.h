@@ -9,7 +9,10 @@
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
 // butterfly argmax across the group; its owner stages it in LDS (double
 // buffered, one barrier per column) and every lane eliminates that column from
-// its rows.  Bubble data needed by the coupling sums (x, H0..H6) and the
+// its rows.  For N <= 4 (n <= 8) the group instead stages its n rows in LDS
+// once and every lane solves the whole system in registers with the lane
+// mapping's LU (identical results on every lane, one barrier instead of n).
+// Bubble data needed by the coupling sums (x, H0..H6) and the
 // coupling constants mx are staged in LDS per system.  All lanes of a group
 // reduce the error ratio and the collision gap to identical values, so the
 // step controller runs redundantly and stays consistent without a broadcast.
@@ -17,6 +20,7 @@
 
 #include "cbc_device.h"
 #include "cbc_kernels.h"
+#include "cbc_linalg.h"
 
 namespace cbc {
 
@@ -27,17 +31,20 @@ struct GroupShape {
     static constexpr int G = pow2_ceil(N);   // lanes per system
     static constexpr int S = 64 / G;         // systems per wave
     static constexpr int n = 2 * N;          // coupling unknowns
+    static constexpr bool redundant = n <= 8;  // every lane solves the whole system
 };
 
 template <int N>
 struct GroupLds {
     static constexpr int G = GroupShape<N>::G;
     static constexpr int n = 2 * N;
+    static constexpr bool R = GroupShape<N>::redundant;
     double mx[3][N][N];        // mx[c][i][j]: effect of bubble j on bubble i
     double xh[8][G];           // x, H0..H6 of every bubble
     double ev[2][G];           // collision radius and position
-    double piv[2][n + 1];      // pivot row, double buffered
-    double sol[n];             // solved accelerations
+    double piv[2][R ? 1 : n + 1];   // pivot row, double buffered (Gauss--Jordan)
+    double sol[R ? 1 : n];          // solved accelerations (Gauss--Jordan)
+    double rows[R ? n : 1][n + 1];  // the augmented system (redundant solve)
 };
 
 // Butterfly reductions inside a group of G lanes (every lane gets the result).
@@ -81,19 +88,51 @@ struct GroupParams {
     Drive<NK> d;
 };
 
+// Harmonics and polytropic factor of the lane's bubble at one stage.
+template <int NK>
+struct GroupStage {
+    double st[NK], ct[NK];
+    double sx[NK], cx[NK];
+    double pf;
+};
+
+// First stage of an attempt, in full; rxb = 1/(R/R0).
+template <int NK, int FIELD>
+__device__ __forceinline__ void group_stage_base(const GroupParams<NK>& P, double t, const double (&y)[4],
+                                                 GroupStage<NK>& B, double& rxb)
+{
+    time_base<NK>(P.d, t, B.st, B.ct);
+    if constexpr (FIELD != kFieldConst) space_base<NK>(P.d, y[1], B.sx, B.cx);
+    B.pf = pow_pos(y[0], -P.sp0);
+    rxb = rcp_fast(y[0]);
+}
+
+// A later stage at (tq, x) from the first stage B at (t, y); `vote` marks
+// lanes that carry a live bubble.
+template <int NK, int FIELD>
+__device__ __forceinline__ void group_stage_next(const GroupParams<NK>& P, const GroupStage<NK>& B, double rxb,
+                                                 double t, const double (&y)[4], double tq, const double (&x)[4],
+                                                 bool vote, GroupStage<NK>& S)
+{
+    time_stage<NK>(P.d, B.st, B.ct, t, tq, vote, S.st, S.ct);
+    if constexpr (FIELD != kFieldConst) {
+        if (vote && space_far<NK>(P.d, y[1], x[1])) space_base<NK>(P.d, x[1], S.sx, S.cx);
+        else space_shift<NK>(P.d, B.sx, B.cx, y[1], x[1], S.sx, S.cx);
+    }
+    if (vote && pf_far(P.sp0, y[0], rxb, x[0])) S.pf = pow_pos(x[0], -P.sp0);
+    else S.pf = pf_shift(P.sp0, B.pf, y[0], rxb, x[0]);
+}
+
 // Coupled RHS for the lane's bubble i.  y, dy = (R, x, U, V) of bubble i.
 template <int N, int NK, int FIELD>
-__device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>& L, int i, bool vu, double t,
-                                          const double (&y)[4], double (&dy)[4])
+__device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>& L, int i, bool vu,
+                                          const GroupStage<NK>& S, const double (&y)[4], double (&dy)[4])
 {
     constexpr int n = 2 * N;
-    double st[NK], ct[NK];
-#pragma unroll
-    for (int k = 0; k < NK; ++k) sincos_fast(P.d.wt[k] * t + P.d.ph[k], &st[k], &ct[k]);
     double pa, pat, pax, uax;
-    field_at<NK, FIELD>(P.d, st, ct, y[1], pa, pat, pax, uax);
+    field_from<NK, FIELD>(P.d, S.st, S.ct, S.sx, S.cx, pa, pat, pax, uax);
     BubbleOut o;
-    bubble_rhs(P.up, P.sp0, P.sp3, y[0], y[2], y[3], pa, pat, pax, uax, o);
+    bubble_rhs(P.up, P.sp3, y[0], y[2], y[3], pa, pat, pax, uax, S.pf, o);
     dy[0] = y[2];
     dy[1] = y[3];
     if constexpr (N == 1) {
@@ -145,6 +184,32 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         ar[n] = o.dU + (rg0 * o.G0 + rg1 * o.G1);
         at[n] = o.dV + (tng + tg2 * o.G2 + tg3 * o.G3);
 
+        if constexpr (GroupShape<N>::redundant) {
+            // stage the n rows, then every lane solves the whole system in VGPRs
+            if (vu) {
+#pragma unroll
+                for (int j = 0; j <= n; ++j) {
+                    L.rows[i][j] = ar[j];
+                    L.rows[N + i][j] = at[j];
+                }
+            }
+            __syncthreads();
+            double a[n][n + 1];
+#pragma unroll
+            for (int r = 0; r < n; ++r)
+#pragma unroll
+                for (int j = 0; j <= n; ++j) a[r][j] = L.rows[r][j];
+            double v[n];
+            lu_solve<n>(a, v);
+            double d2 = v[0], d3 = v[N];
+#pragma unroll
+            for (int j = 1; j < N; ++j) {
+                d2 = (j == ii) ? v[j] : d2;
+                d3 = (j == ii) ? v[N + j] : d3;
+            }
+            dy[2] = d2;
+            dy[3] = d3;
+        } else {
         // Gauss--Jordan with partial pivoting across the group
         bool used_r = !vu, used_t = !vu;
         int col_r = 0, col_t = 0;
@@ -195,6 +260,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         __syncthreads();
         dy[2] = L.sol[ii];
         dy[3] = L.sol[N + ii];
+        }
     }
 }
 

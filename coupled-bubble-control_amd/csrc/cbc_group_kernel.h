@@ -186,32 +186,54 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
         if (h > remain) h = remain;
     }
 
-    double k1[4], k2[4], k3[4], k4[4], k5[4], k6[4], tmp[4];
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t, y, k1);
+    // stage loop as in lane_attempt (cbc_lane.hip)
+    double acc[6][4], ysum[4], esum[4], x[4], k[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a2 * h, tmp, k2);
+    for (int c = 0; c < 4; ++c) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a3 * h, tmp, k3);
+        for (int r = 1; r < 6; ++r) acc[r][c] = 0.0;
+        ysum[c] = 0.0;
+        esum[c] = 0.0;
+    }
+    GroupStage<NK> B, S;
+    double rxb;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a4 * h, tmp, k4);
+    for (int q = 0; q < 6; ++q) {
+        if (q == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a5 * h, tmp, k5);
+            for (int c = 0; c < 4; ++c) x[c] = y[c];
+            group_stage_base<NK, FIELD>(R.P, t, y, B, rxb);
+            S = B;
+        } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
-                             CK::b65 * k5[c]);
-    group_rhs<N, NK, FIELD>(R.P, L, i, vu, t + CK::a6 * h, tmp, k6);
+            for (int r = 1; r < 6; ++r) {
+                if (r == q) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) x[c] = y[c] + h * acc[r][c];
+                }
+            }
+            group_stage_next<NK, FIELD>(R.P, B, rxb, t, y, t + kCkA[q] * h, x, vu, S);
+        }
+        group_rhs<N, NK, FIELD>(R.P, L, i, vu, S, x, k);
+#pragma unroll
+        for (int r = 1; r < 6; ++r) {
+            if (r > q) {
+                const double b = kCkB[r][q];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] += b * k[c];
+            }
+        }
+        const double cq = kCkC[q], eq = kCkE[q];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ysum[c] += cq * k[c];
+            esum[c] += eq * k[c];
+        }
+    }
 
     double ynew[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
-                              CK::c5 * k5[c] + CK::c6 * k6[c]);
+    for (int c = 0; c < 4; ++c) ynew[c] = y[c] + h * ysum[c];
     const double g_new = group_event<N>(L, i, vu, R.P.rcol, ynew, a.col_factor);
 
     // error ratio of the lane's bubble, reduced over the group
@@ -220,8 +242,7 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     if (vu) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
-                                       CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
+            const double e = fabs(h * esum[c]) + 1.0e-18;
             const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
             const double r = tol * rcp_fast(e);
             bad_u |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);

@@ -9,6 +9,7 @@
 // and synchronises the block ~20 times per step; here a system never leaves
 // its lane.
 #include "cbc_device.h"
+#include "cbc_linalg.h"
 #include "cbc_kernels.h"
 #include "cbc_serve.h"
 
@@ -23,65 +24,71 @@ struct LaneSys {
     Drive<NK> d;
 };
 
-// In-register LU with partial pivoting of the augmented n x (n+1) system.
-// The pivot search is a sequence of conditional row exchanges so every index
-// stays a compile-time constant (no scratch).  The first row holding the
-// largest |a_ik| ends up as pivot, as in the CPU restatement.  The coupling
-// matrix is I + C with |C| small, so exchanges are rare: the exchange code
-// runs only when some lane of the wave needs one (wave-uniform branch).
-template <int n>
-__device__ __forceinline__ void lu_solve(double (&a)[n][n + 1], double (&x)[n])
+// Harmonics and polytropic factors of one stage of one system.
+template <int N, int NK>
+struct LaneStage {
+    double st[NK], ct[NK];        // time harmonics (shared by the bubbles)
+    double sx[N][NK], cx[N][NK];  // space harmonics per bubble
+    double pf[N];                 // (R/R0)^(-3 kappa) per bubble
+};
+
+// First stage of an attempt, in full; rxb = 1/(R/R0).
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ void lane_stage_base(const LaneSys<N, NK>& P, double t, const double (&y)[4 * N],
+                                                LaneStage<N, NK>& B, double (&rxb)[N])
 {
-    double inv[n];
+    time_base<NK>(P.d, t, B.st, B.ct);
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-        bool need = false;
-#pragma unroll
-        for (int i = k + 1; i < n; ++i) need |= fabs(a[i][k]) > fabs(a[k][k]);
-        if (__any(need))
-#pragma unroll
-        for (int i = k + 1; i < n; ++i) {
-            const bool sw = fabs(a[i][k]) > fabs(a[k][k]);
-#pragma unroll
-            for (int j = k; j <= n; ++j) {
-                const double u = a[k][j];
-                const double v = a[i][j];
-                a[k][j] = sw ? v : u;
-                a[i][j] = sw ? u : v;
-            }
-        }
-        inv[k] = rcp_fast(a[k][k]);
-#pragma unroll
-        for (int i = k + 1; i < n; ++i) {
-            const double f = a[i][k] * inv[k];
-#pragma unroll
-            for (int j = k + 1; j <= n; ++j) a[i][j] -= f * a[k][j];
-        }
-    }
-#pragma unroll
-    for (int i = n - 1; i >= 0; --i) {
-        double s = a[i][n];
-#pragma unroll
-        for (int j = i + 1; j < n; ++j) s -= a[i][j] * x[j];
-        x[i] = s * inv[i];
+    for (int i = 0; i < N; ++i) {
+        if constexpr (FIELD != kFieldConst) space_base<NK>(P.d, y[N + i], B.sx[i], B.cx[i]);
+        B.pf[i] = pow_pos(y[i], -P.sp0);
+        rxb[i] = rcp_fast(y[i]);
     }
 }
 
-// Full coupled RHS of one system; y, dy ordered [R(N), x(N), U(N), V(N)].
+// A later stage at (tq, x) from the first stage B at (t, y).
 template <int N, int NK, int FIELD>
-__device__ __forceinline__ void lane_rhs(const LaneSys<N, NK>& P, double t, const double (&y)[4 * N],
-                                         double (&dy)[4 * N])
+__device__ __forceinline__ void lane_stage_next(const LaneSys<N, NK>& P, const LaneStage<N, NK>& B,
+                                                const double (&rxb)[N], double t, const double (&y)[4 * N],
+                                                double tq, const double (&x)[4 * N], LaneStage<N, NK>& S)
 {
-    double st[NK], ct[NK];
+    time_stage<NK>(P.d, B.st, B.ct, t, tq, true, S.st, S.ct);
+    if constexpr (FIELD != kFieldConst) {
+        bool far = false;
 #pragma unroll
-    for (int k = 0; k < NK; ++k) sincos_fast(P.d.wt[k] * t + P.d.ph[k], &st[k], &ct[k]);
+        for (int i = 0; i < N; ++i) far |= space_far<NK>(P.d, y[N + i], x[N + i]);
+        if (far) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) space_base<NK>(P.d, x[N + i], S.sx[i], S.cx[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) space_shift<NK>(P.d, B.sx[i], B.cx[i], y[N + i], x[N + i], S.sx[i], S.cx[i]);
+        }
+    }
+    bool far = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) far |= pf_far(P.sp0, y[i], rxb[i], x[i]);
+    if (far) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) S.pf[i] = pow_pos(x[i], -P.sp0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) S.pf[i] = pf_shift(P.sp0, B.pf[i], y[i], rxb[i], x[i]);
+    }
+}
 
+// Full coupled RHS of one system from its stage values; y, dy ordered
+// [R(N), x(N), U(N), V(N)].
+template <int N, int NK, int FIELD>
+__device__ __forceinline__ void lane_rhs(const LaneSys<N, NK>& P, const LaneStage<N, NK>& S,
+                                         const double (&y)[4 * N], double (&dy)[4 * N])
+{
     BubbleOut o[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         double pa, pat, pax, uax;
-        field_at<NK, FIELD>(P.d, st, ct, y[N + i], pa, pat, pax, uax);
-        bubble_rhs(P.up[i], P.sp0, P.sp3, y[i], y[2 * N + i], y[3 * N + i], pa, pat, pax, uax, o[i]);
+        field_from<NK, FIELD>(P.d, S.st, S.ct, S.sx[i], S.cx[i], pa, pat, pax, uax);
+        bubble_rhs(P.up[i], P.sp3, y[i], y[2 * N + i], y[3 * N + i], pa, pat, pax, uax, S.pf[i], o[i]);
         dy[i] = y[2 * N + i];
         dy[N + i] = y[3 * N + i];
     }
@@ -349,33 +356,59 @@ __device__ __forceinline__ bool lane_attempt(const LaunchArgs& a, LaneRun<N, NK>
         if (h > remain) h = remain;
     }
 
-    // Cash--Karp stages (rkck45_coupled_numba_stepper.py:379-484)
-    double k1[4 * N], k2[4 * N], k3[4 * N], k4[4 * N], k5[4 * N], k6[4 * N], tmp[4 * N];
-    lane_rhs<N, NK, FIELD>(P, t, y, k1);
+    // Cash--Karp stages (rkck45_coupled_numba_stepper.py:379-484): k_q goes at
+    // once into the inputs of the later stages (acc[r]), the 5th-order sum and
+    // the error sum, in the reference's order of summation.  The loop is
+    // unrolled: a rolled loop (one copy of the RHS, smaller code) measured
+    // slower on gfx950 because the accumulators then stay live across the
+    // loop and spill to AGPRs.
+    double acc[6][4 * N], ysum[4 * N], esum[4 * N], x[4 * N], k[4 * N];
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b21 * k1[c]);
-    lane_rhs<N, NK, FIELD>(P, t + CK::a2 * h, tmp, k2);
+    for (int c = 0; c < 4 * N; ++c) {
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b31 * k1[c] + CK::b32 * k2[c]);
-    lane_rhs<N, NK, FIELD>(P, t + CK::a3 * h, tmp, k3);
+        for (int r = 1; r < 6; ++r) acc[r][c] = 0.0;
+        ysum[c] = 0.0;
+        esum[c] = 0.0;
+    }
+    LaneStage<N, NK> B, S;
+    double rxb[N];
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c) tmp[c] = y[c] + h * (CK::b41 * k1[c] + CK::b42 * k2[c] + CK::b43 * k3[c]);
-    lane_rhs<N, NK, FIELD>(P, t + CK::a4 * h, tmp, k4);
+    for (int q = 0; q < 6; ++q) {
+        if (q == 0) {
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c)
-        tmp[c] = y[c] + h * (CK::b51 * k1[c] + CK::b52 * k2[c] + CK::b53 * k3[c] + CK::b54 * k4[c]);
-    lane_rhs<N, NK, FIELD>(P, t + CK::a5 * h, tmp, k5);
+            for (int c = 0; c < 4 * N; ++c) x[c] = y[c];
+            lane_stage_base<N, NK, FIELD>(P, t, y, B, rxb);
+            S = B;
+        } else {
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c)
-        tmp[c] = y[c] + h * (CK::b61 * k1[c] + CK::b62 * k2[c] + CK::b63 * k3[c] + CK::b64 * k4[c] +
-                             CK::b65 * k5[c]);
-    lane_rhs<N, NK, FIELD>(P, t + CK::a6 * h, tmp, k6);
+            for (int r = 1; r < 6; ++r) {
+                if (r == q) {
+#pragma unroll
+                    for (int c = 0; c < 4 * N; ++c) x[c] = y[c] + h * acc[r][c];
+                }
+            }
+            lane_stage_next<N, NK, FIELD>(P, B, rxb, t, y, t + kCkA[q] * h, x, S);
+        }
+        lane_rhs<N, NK, FIELD>(P, S, x, k);
+#pragma unroll
+        for (int r = 1; r < 6; ++r) {
+            if (r > q) {
+                const double b = kCkB[r][q];
+#pragma unroll
+                for (int c = 0; c < 4 * N; ++c) acc[r][c] += b * k[c];
+            }
+        }
+        const double cq = kCkC[q], eq = kCkE[q];
+#pragma unroll
+        for (int c = 0; c < 4 * N; ++c) {
+            ysum[c] += cq * k[c];
+            esum[c] += eq * k[c];
+        }
+    }
 
     double ynew[4 * N];
 #pragma unroll
-    for (int c = 0; c < 4 * N; ++c)
-        ynew[c] = y[c] + h * (CK::c1 * k1[c] + CK::c2 * k2[c] + CK::c3 * k3[c] + CK::c4 * k4[c] +
-                              CK::c5 * k5[c] + CK::c6 * k6[c]);
+    for (int c = 0; c < 4 * N; ++c) ynew[c] = y[c] + h * ysum[c];
     const double g_new = lane_event<N, NK>(P, ynew, a.col_factor);
 
     // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
@@ -395,8 +428,7 @@ __device__ __forceinline__ bool lane_attempt(const LaunchArgs& a, LaneRun<N, NK>
         bool bad = false;
 #pragma unroll
         for (int c = 0; c < 4 * N; ++c) {
-            const double e = fabs(h * (CK::e1 * k1[c] + CK::e2 * k2[c] + CK::e3 * k3[c] + CK::e4 * k4[c] +
-                                       CK::e5 * k5[c] + CK::e6 * k6[c])) + 1.0e-18;
+            const double e = fabs(h * esum[c]) + 1.0e-18;
             const double tol = fmax(a.rtol * fmax(fabs(ynew[c]), fabs(y[c])), a.atol);
             const double r = tol * rcp_fast(e);
             bad |= (r != r) || !(fabs(ynew[c]) <= 1.7976931348623157e308);

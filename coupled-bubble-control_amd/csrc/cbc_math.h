@@ -19,8 +19,17 @@
 //   pow_pos x^y = exp(y * log x) for x > 0.
 //   rcp     hardware reciprocal estimate plus two Newton--Raphson steps.
 //
-// Accuracy (tested against the host libm in tests): sincos within 2 ulp,
-// exp/log within 2 ulp, pow_pos within 1e-15 relative for |y log x| < 40.
+// Inside one Cash--Karp attempt the stage arguments stay close to those of
+// the attempt's first stage, so the stages reuse the first stage's values:
+//   sincos_kern  the fdlibm kernels alone (|r| <= pi/4), as sin r, cos r - 1;
+//   sincos_tiny  Taylor series of sin b, cos b - 1 for |b| <= 1/20;
+//   rotate       sin/cos(A + b) from sin/cos A by the addition theorem;
+//   pow1p_small  (1 + e)^y = exp(y log1p e) by series for |e| <= 1/100, |y| <= 6.
+//
+// Accuracy against numpy float64 is checked on the device by
+// cbc_selftest_math (tests/test_gpu_math.py): sincos, exp, log within 2 ulp,
+// rcp within 1 ulp, pow_pos within (|y log x| + 4) ulp relative (the log's
+// rounding scaled by the exponent).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -137,5 +146,71 @@ __device__ __forceinline__ double log_fast(double x)
 
 // x^y for x > 0
 __device__ __forceinline__ double pow_pos(double x, double y) { return exp_fast(y * log_fast(x)); }
+
+constexpr double kPi4 = 0.78539816339744830962;  // bound of sincos_kern
+constexpr double kTinyAngle = 0.05;              // bound of sincos_tiny
+constexpr double kSmallRatio = 0.01;             // bound of pow1p_small (|e|)
+constexpr double kSmallExp = 6.0;                // bound of pow1p_small (|y|)
+
+// sin r and cos r - 1 for |r| <= pi/4: the fdlibm __kernel_sin / __kernel_cos
+// polynomials of sincos_fast without the argument reduction.
+__device__ __forceinline__ void sincos_kern(double r, double* s, double* cm1)
+{
+    const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+    const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+    const double z = r * r;
+    const double v = z * r;
+    const double ps = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    *s = r + v * (S1 + z * ps);
+    const double pc = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    *cm1 = z * pc - 0.5 * z;
+}
+
+// sin b and cos b - 1 for |b| <= 1/20 by Taylor series to b^7 and b^8; the
+// truncation is below 2^-62 relative.
+__device__ __forceinline__ void sincos_tiny(double b, double* s, double* cm1)
+{
+    const double z = b * b;
+    *s = b + (b * z) * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0)));
+    *cm1 = z * (-0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0))));
+}
+
+// sin/cos(A + b) = sin A + (sin A (cos b - 1) + cos A sin b),
+//                  cos A + (cos A (cos b - 1) - sin A sin b)
+__device__ __forceinline__ void rotate(double sa, double ca, double sb, double cm1, double* so, double* co)
+{
+    *so = sa + __builtin_fma(ca, sb, sa * cm1);
+    *co = ca + __builtin_fma(-sa, sb, ca * cm1);
+}
+
+// (1 + e)^y for |e| <= 1/100 and |y| <= 6: log1p by its series to e^8, exp of
+// |y log1p e| <= 0.061 by Taylor to degree 10; truncation below 2^-60 relative.
+__device__ __forceinline__ double pow1p_small(double e, double y)
+{
+    double l = -1.0 / 8.0;
+    l = __builtin_fma(l, e, 1.0 / 7.0);
+    l = __builtin_fma(l, e, -1.0 / 6.0);
+    l = __builtin_fma(l, e, 1.0 / 5.0);
+    l = __builtin_fma(l, e, -1.0 / 4.0);
+    l = __builtin_fma(l, e, 1.0 / 3.0);
+    l = __builtin_fma(l, e, -1.0 / 2.0);
+    l = __builtin_fma(l, e, 1.0);
+    const double u = y * (l * e);
+    double p = 1.0 / 3628800.0;                 // 1/10!
+    p = __builtin_fma(p, u, 1.0 / 362880.0);
+    p = __builtin_fma(p, u, 1.0 / 40320.0);
+    p = __builtin_fma(p, u, 1.0 / 5040.0);
+    p = __builtin_fma(p, u, 1.0 / 720.0);
+    p = __builtin_fma(p, u, 1.0 / 120.0);
+    p = __builtin_fma(p, u, 1.0 / 24.0);
+    p = __builtin_fma(p, u, 1.0 / 6.0);
+    p = __builtin_fma(p, u, 0.5);
+    p = __builtin_fma(p, u, 1.0);
+    return __builtin_fma(p, u, 1.0);
+}
 
 }  // namespace cbc

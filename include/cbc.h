@@ -282,6 +282,29 @@ int32_t cbc_serve_stats(const cbc_serve *serve, int64_t *launches, uint32_t *pub
  * the handle; the device buffers keep every finished item's state. */
 int32_t cbc_serve_destroy(cbc_serve *serve);
 
+/* ---- diagnostics ---------------------------------------------------------
+ * Evaluates the kernels' elementary functions on n device inputs (x, and y for
+ * two-argument functions) into out[2 n] (value pairs (u, v) per input):
+ *   CBC_MATH_SINCOS       (sin x, cos x)
+ *   CBC_MATH_EXP / LOG / RCP   (e^x / log x / 1/x, 0)
+ *   CBC_MATH_POW          (x^y for x > 0, 0)
+ *   CBC_MATH_SINCOS_KERN  (sin x, cos x - 1) for |x| <= pi/4
+ *   CBC_MATH_SINCOS_TINY  (sin x, cos x - 1) for |x| <= 1/20
+ *   CBC_MATH_ROTATE       (sin(x + y), cos(x + y)) by the addition theorem, |y| <= 1/20
+ *   CBC_MATH_POW1P        ((1 + x)^y, 0) for |x| <= 1/100, |y| <= 6
+ * No counterpart in the reference (which calls the CUDA/Numba libm); it lets
+ * tests pin the device arithmetic against a host library.  Asynchronous. */
+#define CBC_MATH_SINCOS 0
+#define CBC_MATH_EXP 1
+#define CBC_MATH_LOG 2
+#define CBC_MATH_POW 3
+#define CBC_MATH_RCP 4
+#define CBC_MATH_SINCOS_KERN 5
+#define CBC_MATH_SINCOS_TINY 6
+#define CBC_MATH_ROTATE 7
+#define CBC_MATH_POW1P 8
+int32_t cbc_selftest_math(int32_t fn, const double *x, const double *y, double *out, int32_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
