@@ -50,6 +50,10 @@ sys.path.insert(0, str(ROOT / "coupled-bubble-control_amd"))
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
+# HBM bytes per system per 10-period launch of the parameter-study kernel, measured with
+# rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a copy kernel of known
+# size): 528 B read + 100 B written (profiles/lane_kernel_param_study_65536xN2.md)
+PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM = 628
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
@@ -108,32 +112,38 @@ def attempts_of(models) -> dict:
 
 
 def cpu_step_sample(W, ws, target_s: float, label: str):
-    """C oracle (OpenMP) on growing subsets of the given workloads until one run takes
-    about target_s / 4, then scaled to target_s; returns the baseline dict."""
+    """C oracle (OpenMP) on growing subsets of the given workloads (the whole set at most,
+    then repeated) until one pass takes at least target_s / 4; returns the baseline dict."""
     import numpy as np
 
     from oracle import ckm1d_oracle as O
 
     nthreads = cpu_threads()
-    frac = 1.0 / 512
+    frac, reps = 1.0 / 512, 1
     while True:
         periods, wall, systems = 0.0, 0.0, 0
-        for w in ws:
-            k = max(8, int(w.ns * frac))
-            sub = w.subset(np.linspace(0, w.ns - 1, k).astype(int))
-            m = W.build_model(sub)
-            bufs = O.host_buffers_dict(m)
-            spec = O.spec_from_model(m)
-            c0 = time.perf_counter()
-            O.solve(spec, bufs, kernel_steps=1 << 20, max_steps=1 << 22)
-            wall += time.perf_counter() - c0
-            periods += float(np.sum(bufs["actual_time"]))
-            systems += k
-        if wall >= target_s / 4 or frac >= 1.0:
+        for _ in range(reps):
+            for w in ws:
+                k = max(8, int(w.ns * frac))
+                sub = w.subset(np.linspace(0, w.ns - 1, k).astype(int))
+                m = W.build_model(sub)
+                bufs = O.host_buffers_dict(m)
+                spec = O.spec_from_model(m)
+                c0 = time.perf_counter()
+                O.solve(spec, bufs, kernel_steps=1 << 20, max_steps=1 << 22)
+                wall += time.perf_counter() - c0
+                periods += float(np.sum(bufs["actual_time"]))
+                systems += k
+        if wall >= target_s / 4:
             break
-        frac = min(1.0, frac * min(8.0, max(2.0, target_s / max(wall, 1e-3))))
+        grow = min(8.0, max(2.0, target_s / max(wall, 1e-3)))
+        if frac < 1.0:
+            frac = min(1.0, frac * grow)
+        else:
+            reps = int(np.ceil(reps * grow))
     return {"value": periods / wall, "unit": "periods/s", "cores": nthreads, "kind": "port",
-            "sample": f"{systems} systems of {label} ({periods:.0f} periods), C oracle with OpenMP, {wall:.1f} s"}
+            "sample": f"{systems} system runs of {label} ({periods:.0f} periods), C oracle with OpenMP, "
+                      f"{wall:.1f} s"}
 
 
 # ---------------------------------------------------------------------------
@@ -326,8 +336,11 @@ def bench_param_study(args):
                    "window_stats": "max R/R0 and mean x per period, periods 51..200",
                    "mapping": info.mapping, "block_size": info.block_size, "vgprs": info.num_regs,
                    "scratch_bytes": info.local_bytes},
-        "roofline": roofline(attempts * F, sum(kernel_ms) * 1e-3, window="kernel (HIP events per launch)",
-                             flops_per_attempt=F, attempts=int(attempts), kernel_ms_mean=float(np.mean(kernel_ms))),
+        "roofline": roofline(attempts * F / steps, float(np.mean(kernel_ms)) * 1e-3,
+                             window="kernel (HIP events per launch)", flops_per_attempt=F, attempts=int(attempts),
+                             kernel_ms_mean=float(np.mean(kernel_ms)),
+                             traffic=PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM * ns,
+                             traffic_note="HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE"),
         "cpu_baseline": cpu,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(flags, return_counts=True))},
     }
