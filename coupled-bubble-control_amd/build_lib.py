@@ -35,11 +35,17 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",
 ]
+# Group-mapping kernels only: -disable-machine-licm keeps the backend from
+# hoisting the step's FP64 constants out of the step loop into SGPR pairs that
+# then spill to VGPR lanes (3-5 % faster per attempt for N = 2..4; the lane
+# kernel measured no gain).
+EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"]}
 
 
 def _compile(src: Path, verbose: bool) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", str(src), "-o", str(obj)]
+    extra = [f for prefix, fl in EXTRA_FLAGS.items() if src.name.startswith(prefix) for f in fl]
+    cmd = [HIPCC, *CFLAGS, *extra, "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -266,3 +266,54 @@ def test_serving_loop_with_resets_runs():
     for m in bins.values():
         y = m.runtime.device_buffers.state.actual_state.cpu().numpy()
         assert np.isfinite(y).all()
+
+
+def test_served_vec_env_equals_blocking_env_step():
+    """Three env steps through ServedVecEnv give the observations and failures of the
+    reference's blocking step (set_action, model.solve, observe, check_termination,
+    advance_time, reset_envs) on the plain solver, bit for bit."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.envs import ServedVecEnv
+    from coupledbubble_control.workloads import sample_positions
+
+    print("\n[served env] N=2 ns=256, 3 steps with resets", flush=True)
+    w, ms, mp = _models(2, 256, seed=17)
+    lam = workloads.wavelength(w.mat, 25.0)
+    gap_um = 3.0 * float(w.R0.max())
+
+    def scene(k, rng):
+        return sample_positions(rng, k, 2, 0.0, 0.25, np.full(k, gap_um * 1e-6 / lam))
+
+    env = ServedVecEnv(ms, time_step_length=2.0, scene=scene, seed=4)
+    rng_ref = np.random.default_rng(4)
+    rng_a = np.random.default_rng(9)
+    mp._ensure_buffers()
+    db = mp.runtime.device_buffers
+    db.state.time_end.fill_(2.0)
+    total_fail = 0
+    try:
+        for step in range(3):
+            pa = rng_a.uniform(0.0, 1.5, size=(256, 2))
+            obs, fail = env.step(pa)
+            # the reference step on the plain path
+            db.params.control_params[0:2].copy_(torch.from_numpy(np.ascontiguousarray(pa.T * 1.0e5)).cuda())
+            mp.solve(max_steps=1 << 20, kernel_steps=1 << 20)
+            ref_obs = db.state.actual_state[1, :, :2].to(torch.float32).cpu().numpy()
+            ref_fail = (db.status.status_flags != 1).cpu().numpy()
+            db.status.status_flags.zero_()
+            db.state.actual_time.zero_()
+            db.state.time_step.fill_(1e-6)
+            db.status.actual_event.fill_(-1.0)
+            ids = np.nonzero(ref_fail)[0]
+            if ids.size:
+                mp.reset_systems(ids, None, scene(ids.size, rng_ref), h0=1e-6)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(fail, ref_fail)
+            np.testing.assert_array_equal(obs, ref_obs)
+            total_fail += int(fail.sum())
+            print(f"  step {step}: {int(fail.sum())} failures", flush=True)
+    finally:
+        env.close()
+    assert total_fail > 0   # the reset path was exercised
