@@ -365,7 +365,7 @@ def bench_ensemble(args, large: bool):
     log(f"{name}: autotune")
     tune = autotune(models(), horizon=0.5, limit=4096)
     tuning = {n: r.choice for n, r in tune.items()}
-    log(f"tuning: { {n: (c.variant, c.refill) for n, c in tuning.items()} }")
+    log(f"tuning: { {n: (c.variant, c.refill, c.block or 64) for n, c in tuning.items()} }")
 
     # warm-up: one short ensemble run
     for _ in range(args.warmup):
@@ -424,8 +424,9 @@ def bench_ensemble(args, large: bool):
         "ms_per_step": wall * 1e3 / steps,
         "config": {"workload": name, "bins": {str(w.n): w.ns for w in ws}, "rtol": 1e-9, "atol": 1e-9,
                    "field": "SW_N", "drive_khz": [25.0, 37.5] if large else [25.0, 50.0],
-                   "tuning": {str(n): {"mapping": c.variant, "refill": c.refill, "waves": c.waves}
-                              for n, c in tuning.items()}},
+                   "tuning": {str(n): {"mapping": c.variant, "refill": c.refill, "waves": c.waves,
+                                       "block": c.block or 64} for n, c in tuning.items()},
+                   "tuning_trials_s": {str(n): r.trials for n, r in tune.items()}},
         "vs_plain": (periods / wall) / (pperiods / pwall),
         "plain_periods_per_s": pperiods / pwall,
         "roofline": roofline(flops / steps, wall / steps, window="timed region (concurrent persistent launches)",

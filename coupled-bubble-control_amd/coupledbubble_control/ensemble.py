@@ -13,7 +13,7 @@ in the reference buffer layout) and integrates all of them together:
   SIMDs for the others (the tails overlap);
 * the launch sequence can be captured once into a HIP graph and replayed;
 * ``autotune`` picks, per bin, the mapping (lane vs group), the refill
-  threshold and the persistent grid from short timed trials.
+  threshold and the lane mapping's workgroup size from short timed trials.
 """
 from __future__ import annotations
 
@@ -36,6 +36,7 @@ class BinTuning:
     variant: str = "auto"      # "lane" | "group" | "auto"
     refill: int = 1            # free lanes (groups) per wave that trigger a refill
     waves: int = 0             # persistent grid in waves (0 = fill every SIMD)
+    block: int = 0             # threads per workgroup of the lane mapping (0 = 64)
 
 
 @dataclass
@@ -61,9 +62,12 @@ class MixedEnsemble:
     def _prepare(self):
         for m in self.models:
             m._ensure_buffers()
-            want = KernelVariant(self.tuning[m._execution_spec.ups].variant)
-            if m.runtime.kernel is None or m.runtime.variant != want:
+            t = self.tuning[m._execution_spec.ups]
+            want = KernelVariant(t.variant)
+            block = t.block or None
+            if m.runtime.kernel is None or m.runtime.variant != want or m._hip_opts.block_size != block:
                 m.runtime.variant = want
+                m._hip_opts.block_size = block
                 m.runtime.solver = m._make_solver(m.runtime.backend, want)
         if len(self._streams) != len(self.models):
             self._streams = [torch.cuda.Stream() for _ in self.models]
@@ -147,32 +151,35 @@ def _trial_copy(m, horizon: float, limit: int):
 
 
 def autotune(models: Sequence, horizon: float = 0.5, limit: int = 8192,
-             refills: Sequence[int] = (1, 16, 64), cache: Optional[Path] = None) -> Dict[int, TuneRecord]:
-    """Pick mapping and refill threshold per bin from timed trials of `horizon` periods on
-    up to `limit` systems of that bin (the group mapping is the only choice for N >= 3
-    in this build; the lane mapping spills beyond N = 2).  Results are cached as JSON
-    keyed by bubble count when `cache` is given."""
+             refills: Sequence[int] = (1, 16, 64), blocks: Sequence[int] = (64, 256),
+             cache: Optional[Path] = None) -> Dict[int, TuneRecord]:
+    """Pick mapping, refill threshold and (lane mapping) workgroup size per bin from timed
+    trials of `horizon` periods on up to `limit` systems of that bin.  The lane mapping is
+    tried for N <= 2 only (it spills beyond).  Results are cached as JSON keyed by bubble
+    count when `cache` is given."""
     if cache is not None and Path(cache).exists():
         data = json.loads(Path(cache).read_text())
         return {int(k): TuneRecord(v["n"], BinTuning(**v["choice"]), v["trials"]) for k, v in data.items()}
     out: Dict[int, TuneRecord] = {}
     for m in models:
         n = m._execution_spec.ups
-        variants = ["lane", "group"] if n <= 2 else ["group"]
+        cands = [BinTuning("group", rf, 0, 0) for rf in refills]
+        if n <= 2:
+            cands = [BinTuning("lane", rf, 0, b) for b in blocks for rf in refills] + cands
         best, best_t, trials = None, float("inf"), {}
-        for var in variants:
-            for rf in refills:
-                c = _trial_copy(m, horizon, limit)
-                ens = MixedEnsemble([c], {n: BinTuning(var, rf, 0)})
-                ens._prepare()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                ens.solve()
-                torch.cuda.synchronize()
-                dt = time.perf_counter() - t0
-                trials[f"{var}/refill={rf}"] = dt
-                if dt < best_t:
-                    best, best_t = BinTuning(var, rf, 0), dt
+        for cand in cands:
+            c = _trial_copy(m, horizon, limit)
+            ens = MixedEnsemble([c], {n: cand})
+            ens._prepare()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ens.solve()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            key = f"{cand.variant}/refill={cand.refill}" + (f"/block={cand.block}" if cand.block else "")
+            trials[key] = dt
+            if dt < best_t:
+                best, best_t = cand, dt
         out[n] = TuneRecord(n, best, trials)
     if cache is not None:
         Path(cache).parent.mkdir(parents=True, exist_ok=True)

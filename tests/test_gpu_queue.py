@@ -147,3 +147,34 @@ def test_graph_replay_matches_eager(device):
             got = _snapshot(m)
             for k in ("state", "t", "h", "flags", "steps"):
                 np.testing.assert_array_equal(got[k], wnt[k], err_msg=f"replay {rep}: {k}")
+
+
+def test_autotune_choices_and_tuned_ensemble(device):
+    """The autotuner times every candidate (lane/group, refill, lane workgroup size) and
+    the ensemble run with its choices equals one plain solve per bin bit for bit."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.ensemble import MixedEnsemble, autotune
+
+    print("case: autotune N=2, N=4", flush=True)
+
+    def build(variants=None):
+        return [workloads.build_model(_uneven(n, ns, seed=70 + n, hmax=1.0),
+                                      variant=KernelVariant(variants[n]) if variants else None)
+                for n, ns in ((2, 256), (4, 64))]
+
+    from coupledbubble_control.backends import KernelVariant
+
+    tune = autotune(build(), horizon=0.2, limit=256, refills=(1, 64), blocks=(64, 128))
+    assert set(tune) == {2, 4}
+    assert len(tune[2].trials) == 2 * 2 + 2 and len(tune[4].trials) == 2
+    assert tune[4].choice.variant == "group"
+    assert all(t > 0 for r in tune.values() for t in r.trials.values())
+    models = build()
+    MixedEnsemble(models, {n: r.choice for n, r in tune.items()}).solve()
+    # a plain solve per bin with the mapping the tuner chose (lane and group differ in rounding)
+    for m, mp in zip(models, build({n: r.choice.variant for n, r in tune.items()})):
+        mp.solve(max_steps=10 ** 7, kernel_steps=10 ** 6)
+        sa, sb = _snapshot(m), _snapshot(mp)
+        for k in sa:
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    print(f"  choices { {n: r.choice for n, r in tune.items()} }", flush=True)
