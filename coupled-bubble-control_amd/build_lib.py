@@ -39,7 +39,9 @@ CFLAGS = [
 # hoisting the step's FP64 constants out of the step loop into SGPR pairs that
 # then spill to VGPR lanes (3-5 % faster per attempt for N = 2..4; the lane
 # kernel measured no gain).
-EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"]}
+# The environment kernels round every float32 operation separately, as the
+# elementwise torch kernels of the reference environment do.
+EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"], "cbc_env": ["-ffp-contract=off"]}
 
 
 def _compile(src: Path, verbose: bool) -> Path:

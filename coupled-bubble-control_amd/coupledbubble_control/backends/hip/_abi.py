@@ -100,6 +100,51 @@ class ServeViewC(ctypes.Structure):
                 ("stage_cm", _vp), ("capacity", _i32), ("waves", _i32)]
 
 
+CBC_ENV_MAX_ACTIONS = 16
+CBC_ENV_MAX_PARTS = 8
+CBC_ENV_MAX_BUBBLES = 16
+CBC_ENV_PART_STATE = 0
+CBC_ENV_PART_TARGET = 1
+CBC_ENV_PART_FIXED = 2
+CBC_SCENE_RANDOM = 0
+CBC_SCENE_EQUIDISTANT = 1
+CBC_SCENE_FIXED = 2
+
+
+class EnvSpecC(ctypes.Structure):
+    _fields_ = [("num_envs", _i32), ("num_bubbles", _i32), ("tile", _i32), ("num_control_rows", _i32),
+                ("num_actions", _i32), ("action_row", _i32 * CBC_ENV_MAX_ACTIONS),
+                ("action_scale", _f64 * CBC_ENV_MAX_ACTIONS), ("num_pa", _i32), ("pa_col", _i32 * CBC_ENV_MAX_ACTIONS),
+                ("num_parts", _i32), ("part_kind", _i32 * CBC_ENV_MAX_PARTS), ("part_row", _i32 * CBC_ENV_MAX_PARTS),
+                ("part_stack", _i32 * CBC_ENV_MAX_PARTS), ("part_hist", _i32 * CBC_ENV_MAX_PARTS),
+                ("obs_dim", _i32), ("history_floats", _i64), ("x_part", _i32), ("target_part", _i32),
+                ("apply_termination", _i32), ("episode_length", _i64), ("position_tolerance", ctypes.c_float),
+                ("r_dnorm", ctypes.c_float), ("reward_exp", ctypes.c_float), ("w_distance", ctypes.c_float),
+                ("w_intensity", ctypes.c_float), ("positive_terminal_reward", ctypes.c_float),
+                ("negative_terminal_reward", ctypes.c_float), ("next_time_step", _f64)]
+
+
+class EnvBuffersC(ctypes.Structure):
+    _fields_ = [(name, _vp) for name in (
+        "actual_state", "actual_time", "time_step", "actual_event", "status_flags", "control_params",
+        "actions", "history", "observations", "obs_bias", "obs_rscale", "rewards", "total_rewards",
+        "algo_steps", "failure", "success", "terminated", "time_outs", "done_ids", "workspace")]
+
+
+class SceneSideC(ctypes.Structure):
+    _fields_ = [("method", _i32), ("random_align", _i32), ("base", _f64 * CBC_ENV_MAX_BUBBLES), ("offset", _f64),
+                ("max_start", _f64), ("u_gaps", _vp), ("u_align", _vp)]
+
+
+class EnvSceneC(ctypes.Structure):
+    _fields_ = [("lo", _f64), ("available", _f64), ("min_distance", _f64), ("room_per_gap", _f64),
+                ("extra_gap_limit", _f64), ("bubbles", SceneSideC), ("targets", SceneSideC)]
+
+
+class EnvCountsC(ctypes.Structure):
+    _fields_ = [("done", _i32), ("failures", _i32), ("successes", _i32), ("time_outs", _i32)]
+
+
 # name -> (restype, argtypes); every entry point of include/cbc.h
 SIGNATURES = {
     "cbc_abi_version": (_i32, []),
@@ -124,6 +169,13 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_uint32)]),
     "cbc_serve_destroy": (_i32, [_vp]),
     "cbc_selftest_math": (_i32, [_i32, _vp, _vp, _vp, _i32, _vp]),
+    "cbc_env_workspace_words": (_i64, [_i32]),
+    "cbc_env_set_action": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _vp]),
+    "cbc_env_post": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _i64, _vp,
+                            ctypes.POINTER(EnvCountsC)]),
+    "cbc_env_reset": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), ctypes.POINTER(EnvSceneC),
+                             _vp, _i32, _i64, _vp, _vp, _vp, _vp]),
+    "cbc_env_scene_positions": (_i32, [ctypes.POINTER(EnvSceneC), _i32, _i32, _vp, _vp, _vp]),
 }
 
 

@@ -102,6 +102,16 @@ struct cbc_plan {
     int device;
 };
 
+namespace cbc {
+// Error hook for the library's other translation units (cbc_env.hip): records
+// the message that cbc_last_error() returns on this thread.
+int32_t report_error(int32_t code, const char* msg)
+{
+    g_last_error = msg;
+    return code;
+}
+}  // namespace cbc
+
 extern "C" {
 
 int32_t cbc_abi_version(void) { return CBC_ABI_VERSION; }

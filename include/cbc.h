@@ -282,6 +282,163 @@ int32_t cbc_serve_stats(const cbc_serve *serve, int64_t *launches, uint32_t *pub
  * the handle; the device buffers keep every finished item's state. */
 int32_t cbc_serve_destroy(cbc_serve *serve);
 
+/* ---- vector-environment step ----------------------------------------------
+ * The device side of PosNBC1D.step / reset_envs
+ * (coupledbubble_control/envs/bubble_1D_pos_control.py:503-538, 583-614,
+ * 731-853) as three kernels over the solver buffers plus the environment's own
+ * buffers:
+ *   cbc_env_set_action  set_action (:732-749)
+ *   cbc_env_post        observe, check_termination, get_rewards, advance_time
+ *                       and the done list of final_termination (:756-842)
+ *   cbc_env_reset       final_observation / episode_return / episode_length of
+ *                       the done envs, then reset_envs (:583-614) with scenes
+ *                       from SceneGenerator (:112-249) computed on the device
+ *                       from the generator's uniform draws.
+ * Observations, rewards and targets are float32 as in the reference
+ * (envs/common/buffers.py, spaces.py); state rows are FP64.
+ *
+ * Observation parts follow SpaceSchema's components in order
+ * (envs/common/specs.py:35-44): a STATE part is a row of actual_state kept as
+ * a ring of `stack` observations (RingBuffer, buffers.py:36-83; stack 1 is the
+ * PersistentBuffer), a TARGET part holds the episode's target positions, a
+ * FIXED part is never written (a component that observe() skips; it may be
+ * stacked and then stays an all-zero ring).  history
+ * holds the parts at part_hist offsets (floats): STATE (stack, NE, N), others
+ * (NE, N).  Ring slots advance once per cbc_env_post; `observed` is the number
+ * of cbc_env_post calls made before the call, so slot = observed % stack.
+ * The observation row is the concatenation of the parts, each bubble-major
+ * with its history oldest first (RingBuffer.get_stacked), scaled per column
+ * as Box.scale: (x - bias) * rscale. */
+#define CBC_ENV_MAX_ACTIONS 16
+#define CBC_ENV_MAX_PARTS 8
+#define CBC_ENV_MAX_BUBBLES 16
+#define CBC_ENV_PART_STATE 0
+#define CBC_ENV_PART_TARGET 1
+#define CBC_ENV_PART_FIXED 2
+
+typedef struct cbc_env_spec {
+    int32_t num_envs;            /* NE = NS of the solver buffers */
+    int32_t num_bubbles;         /* N = UPS */
+    int32_t tile;                /* state row stride (TILE) */
+    int32_t num_control_rows;    /* 4 * NK */
+    int32_t num_actions;         /* action columns */
+    int32_t action_row[CBC_ENV_MAX_ACTIONS];   /* control_params row of each column (ActionMapingManager._dev_cols) */
+    double action_scale[CBC_ENV_MAX_ACTIONS];  /* its scale (1e5 for PA, pi for PS) */
+    int32_t num_pa;              /* action columns of the PA component (actuation penalty) */
+    int32_t pa_col[CBC_ENV_MAX_ACTIONS];
+    int32_t num_parts;
+    int32_t part_kind[CBC_ENV_MAX_PARTS];
+    int32_t part_row[CBC_ENV_MAX_PARTS];       /* state row of a STATE part */
+    int32_t part_stack[CBC_ENV_MAX_PARTS];     /* ring depth (1 for TARGET) */
+    int32_t part_hist[CBC_ENV_MAX_PARTS];      /* float offset of the part in history */
+    int32_t obs_dim;             /* sum over parts of N * stack */
+    int64_t history_floats;      /* floats in history (bounds of the part offsets) */
+    int32_t x_part;              /* the STATE part of the positions (X, row 1) */
+    int32_t target_part;         /* the TARGET part (XT) */
+    int32_t apply_termination;   /* success = every |XT - X| <= position_tolerance */
+    int64_t episode_length;
+    float position_tolerance;
+    float r_dnorm;               /* 1 / dnorm */
+    float reward_exp;
+    float w_distance, w_intensity;                          /* reward weights */
+    float positive_terminal_reward, negative_terminal_reward;
+    double next_time_step;       /* time_step after each step (1e-6); <= 0 keeps the last accepted step */
+} cbc_env_spec;
+
+typedef struct cbc_env_buffers {
+    /* solver buffers (cbc_buffers layouts) */
+    double *actual_state;
+    double *actual_time;
+    double *time_step;
+    double *actual_event;
+    int32_t *status_flags;
+    double *control_params;
+    /* environment buffers, (NE,) unless stated */
+    const float *actions;        /* (NE, num_actions) */
+    float *history;              /* observation parts, see above */
+    float *observations;         /* (NE, obs_dim) */
+    const float *obs_bias;       /* (obs_dim,) */
+    const float *obs_rscale;     /* (obs_dim,) */
+    float *rewards;
+    float *total_rewards;
+    int64_t *algo_steps;
+    uint8_t *failure;            /* bool: status != SUCCESS */
+    uint8_t *success;
+    uint8_t *terminated;         /* failure | success */
+    uint8_t *time_outs;          /* algo_steps >= episode_length */
+    int64_t *done_ids;           /* (NE,) ascending ids of terminated or timed-out envs */
+    uint32_t *workspace;         /* cbc_env_workspace_words(NE) words, zeroed once before the first call */
+} cbc_env_buffers;
+
+/* Scene of one side (bubbles or targets), SceneGenerator.generate_new_scenes:
+ * RANDOM: _random_uniform_position_1D (:183-242), draws (n, N-1) uniforms plus
+ *   (n, 1) for random alignment; EQUIDISTANT: _equidistant_position_1D
+ *   (:139-181), base positions `base` plus an offset (random alignment: one
+ *   uniform per scene, offset = lo + u (max_start - lo)); FIXED: `base`. */
+#define CBC_SCENE_RANDOM 0
+#define CBC_SCENE_EQUIDISTANT 1
+#define CBC_SCENE_FIXED 2
+typedef struct cbc_scene_side {
+    int32_t method;
+    int32_t random_align;        /* alignment == "random" */
+    double base[CBC_ENV_MAX_BUBBLES];
+    double offset;               /* EQUIDISTANT, centred or no distance */
+    double max_start;            /* EQUIDISTANT, random alignment */
+    const double *u_gaps;        /* RANDOM: (n, N-1) draws */
+    const double *u_align;       /* random alignment: (n,) draws */
+} cbc_scene_side;
+
+typedef struct cbc_env_scene {
+    double lo;                   /* domain_min + margin */
+    double available;            /* hi - lo */
+    double min_distance;
+    double room_per_gap;         /* max_distance - min_distance */
+    double extra_gap_limit;      /* available - (N-1) min_distance */
+    cbc_scene_side bubbles;
+    cbc_scene_side targets;
+} cbc_env_scene;
+
+/* Counts of the last cbc_env_post (host values). */
+typedef struct cbc_env_counts {
+    int32_t done;
+    int32_t failures;
+    int32_t successes;
+    int32_t time_outs;
+} cbc_env_counts;
+
+/* Words of cbc_env_buffers.workspace for NE environments. */
+int64_t cbc_env_workspace_words(int32_t num_envs);
+
+/* set_action (envs/bubble_1D_pos_control.py:732-749): control_params[action_row[c]][e]
+ * = (double)actions[e][c] * action_scale[c].  Asynchronous. */
+int32_t cbc_env_set_action(const cbc_env_spec *spec, const cbc_env_buffers *buf, void *stream);
+
+/* observe (:756-769), check_termination (:772-789), get_rewards (:792-813),
+ * advance_time (:816-827) and the done list of final_termination (:830-842),
+ * one thread per env, in one launch; the last workgroup to finish writes the
+ * ordered done_ids and the counts.  When `counts` is not NULL the call waits
+ * for the stream and returns them (the reference's host syncs of the same
+ * step); otherwise it is asynchronous. */
+int32_t cbc_env_post(const cbc_env_spec *spec, const cbc_env_buffers *buf, int64_t observed, void *stream,
+                     cbc_env_counts *counts);
+
+/* final_termination's per-done-env outputs and reset_envs (:583-614) for the n
+ * envs ids[0..n): final_observation (n, obs_dim), episode_return (n,),
+ * episode_length (n,) (each may be NULL) are taken before the reset; then new
+ * positions and targets from `scene`, R = 1, X = positions, U = V = 0 in
+ * actual_state, the X ring filled with the positions and the targets stored,
+ * algo_steps = 0, total_rewards = 0, and the observation rows rebuilt.
+ * `observed` counts the cbc_env_post calls made so far.  Asynchronous. */
+int32_t cbc_env_reset(const cbc_env_spec *spec, const cbc_env_buffers *buf, const cbc_env_scene *scene,
+                      const int64_t *ids, int32_t n, int64_t observed, float *final_observation,
+                      float *episode_return, int64_t *episode_length, void *stream);
+
+/* SceneGenerator.generate_new_scenes (:112-137) alone: positions (n, N) and
+ * targets (n, N) of n scenes (either may be NULL), by the arithmetic of
+ * cbc_env_reset.  Asynchronous. */
+int32_t cbc_env_scene_positions(const cbc_env_scene *scene, int32_t n, int32_t num_bubbles, double *positions,
+                                double *targets, void *stream);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Evaluates the kernels' elementary functions on n device inputs (x, and y for
  * two-argument functions) into out[2 n] (value pairs (u, v) per input):
