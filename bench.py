@@ -29,6 +29,12 @@ bubbles with implicit radial/translational coupling):
   ensemble.
 * ``large`` -- 4 096 systems of N=16 (32 x 32 coupling solve per RHS) mixed 1:1
   with 4 096 systems of N=2, 20 periods, rtol = atol = 1e-9; as ``mixed``.
+* ``env`` -- the vector environment itself: PosNBC1D with 16 384 envs of 2
+  bubbles (``--env-bubbles``, ``--env-count``) at its default physics (R0 = 60 um,
+  25 + 50 kHz SW_N drive, 5 periods per step, 128-step episodes, random
+  scenes), random actions; a step is one env.step (set_action, solve, observe,
+  termination, rewards, advance_time, resets of finished episodes).  Reports
+  step latency and the split between the solver and the rest of the step.
 
 Every line carries the FP64 roofline of the dominant kernel (algorithmic FLOPs
 per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
@@ -73,7 +79,7 @@ def flops_per_attempt(n: int) -> int:
 def parse():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large"], default="serving")
+    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env"], default="serving")
     p.add_argument("--steps", type=int, default=None,
                    help="serving: requests timed (default 1100, ~3 s of arrivals); param_study: steps of "
                         "--periods-per-step (default 20); mixed/large: ensemble solves (default 1)")
@@ -82,6 +88,8 @@ def parse():
     p.add_argument("--saturation-s", type=float, default=3.0, help="serving: closed-loop seconds per path (0: off)")
     p.add_argument("--periods-per-step", type=float, default=10.0)
     p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
+    p.add_argument("--env-bubbles", type=int, default=2, help="env: bubbles per env")
+    p.add_argument("--env-count", type=int, default=16384, help="env: number of envs")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample length")
     return p.parse_args()
@@ -436,6 +444,125 @@ def bench_ensemble(args, large: bool):
     }
 
 
+def _env_oracle_sample(n: int, T: float, target_s: float) -> dict:
+    """The oracle environment (oracle/env_step.py) with the C oracle solver on a
+    growing number of envs of the env workload, random actions, until one run of
+    three steps takes at least target_s / 4."""
+    import numpy as np
+
+    from coupledbubble_control.models import CKM1D
+    from oracle import ckm1d_oracle as C
+    from oracle import env_step as E
+
+    k = 16
+    rng = np.random.default_rng(1)
+    while True:
+        m = CKM1D(num_systems=k, systems_per_block=64, units_per_system=n, R0=np.full((k, n), 60.0),
+                  PA=np.tile([[1.0], [0.0]], k), FR=np.tile([[25.0], [50.0]], k), PS=np.zeros((2, k)),
+                  REF_FREQ=25.0, acoustic_field="SW_N")
+        parts = [E.Part("XT", [-0.5] * n, [0.5] * n), E.Part("X", [-0.5] * n, [0.5] * n, stack=2)]
+        env = E.EnvOracle(k, n, parts, action_rows=[0, 1, 4, 5], action_scales=[1e5, 1e5, np.pi, np.pi],
+                          pa_cols=[0, 1])
+        sp = E.SceneParams(num_bubbles=n, domain_min=-0.5, domain_max=0.5)
+        bufs = C.host_buffers_dict(m)
+        bufs["time_end"][:] = T
+        env.reset_envs(np.arange(k), E.random_uniform_positions(sp, rng.random((k, n - 1))),
+                       E.random_uniform_positions(sp, rng.random((k, n - 1))), bufs["actual_state"])
+        spec = C.spec_from_model(m)
+        c0 = time.perf_counter()
+        for _ in range(3):
+            act = rng.random((k, 4)).astype(np.float32)
+            env.set_action(act, bufs["control_params"])
+            C.solve(spec, bufs, kernel_steps=2048, max_steps=10 ** 6)
+            r = env.post(bufs["actual_state"], bufs["actual_time"], bufs["time_step"], bufs["actual_event"],
+                         bufs["status_flags"], act)
+            d = r["done_ids"]
+            if d.size:
+                env.reset_envs(d, E.random_uniform_positions(sp, rng.random((d.size, n - 1))),
+                               E.random_uniform_positions(sp, rng.random((d.size, n - 1))), bufs["actual_state"])
+        wall = time.perf_counter() - c0
+        if wall >= target_s / 4 or k >= 1 << 14:
+            break
+        k = min(1 << 14, int(k * min(8.0, max(2.0, target_s / max(wall, 1e-3)))))
+    return {"value": k * T * 3 / wall, "unit": "periods/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"3 env steps of {k} envs x {n} bubbles (oracle environment, C oracle solver with "
+                      f"OpenMP), {wall:.1f} s"}
+
+
+def bench_env(args):
+    """PosNBC1D steps: latency per env.step and the solver / pipeline split."""
+    import numpy as np
+    import torch
+
+    from coupledbubble_control.envs import PosNBC1D
+
+    n, ne = args.env_bubbles, args.env_count
+    steps = args.steps or 20
+    log(f"env: building PosNBC1D {ne} envs x {n} bubbles")
+    env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False)
+    T = float(env.time_step_length)
+    env.reset()
+    stream = torch.cuda.current_stream()
+    ev = {"physics": [], "post": [], "reset": []}
+
+    def timed(fn, key):
+        def run(*a, **kw):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            r = fn(*a, **kw)
+            e1.record(stream)
+            ev[key].append((e0, e1))
+            return r
+        return run
+
+    env.step_physics = timed(env.step_physics, "physics")
+    env._kernels.post = timed(env._kernels.post, "post")
+    env._kernels.reset = timed(env._kernels.reset, "reset")
+    log(f"env: {args.warmup} warmup steps")
+    for _ in range(args.warmup):
+        env.step(None)
+    torch.cuda.synchronize()
+    for v in ev.values():
+        v.clear()
+    a0 = attempts_of([env.model])[n]
+    lat, dones = [], 0
+    log(f"env: {steps} timed steps")
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s0 = time.perf_counter()
+        info = env.step(None)[4]
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - s0)
+        dones += int(info["dones"].numel()) if info else 0
+    wall = time.perf_counter() - t0
+    att = attempts_of([env.model])[n] - a0
+
+    def ms(key):
+        return [e0.elapsed_time(e1) for e0, e1 in ev[key]]
+
+    phys, post, rst = ms("physics"), ms("post"), ms("reset")
+    lat_ms = np.array(lat) * 1e3
+    flops = float(att) * flops_per_attempt(n)
+    out = {
+        "value": ne * T * steps / wall, "unit": "periods/s", "steps": steps, "ms_per_step": wall * 1e3 / steps,
+        "config": {"workload": f"env: PosNBC1D {ne} envs x {n} bubbles, {T:g} periods per step, random actions",
+                   "envs": ne, "bubbles": n, "periods_per_step": T},
+        "env_steps_per_s": ne * steps / wall,
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "split_ms_per_step": {"solver": float(np.mean(phys)), "post_kernel_and_counts": float(np.mean(post)),
+                              "reset": float(np.mean(rst)) if rst else 0.0,
+                              "host_rest": float(np.mean(lat_ms) - np.mean(phys) - np.mean(post)
+                                                 - (np.sum(rst) / steps if rst else 0.0))},
+        "episodes_finished": dones,
+        "roofline": roofline(flops, float(np.sum(phys)) / 1e3, window="solver launches of the timed steps",
+                             attempts={str(n): att}),
+    }
+    if args.cpu_baseline:
+        log("env: CPU baseline (oracle environment)")
+        out["cpu_baseline"] = _env_oracle_sample(n, T, args.cpu_seconds)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -450,6 +577,8 @@ def main():
         out = bench_serving(args)
     elif args.workload == "param_study":
         out = bench_param_study(args)
+    elif args.workload == "env":
+        out = bench_env(args)
     else:
         out = bench_ensemble(args, large=args.workload == "large")
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,

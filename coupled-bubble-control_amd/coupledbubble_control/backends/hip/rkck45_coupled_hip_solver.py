@@ -81,9 +81,11 @@ def _check_trig_range(d_buffers, model_spec):
     nk = model_spec.nk
     cp = d_buffers.params.control_params
     sp = d_buffers.params.system_params
-    wmax = (cp[nk:2 * nk].abs() * sp[1:2].abs()).max().item() * 2.0 * np.pi
-    tmax = torch.maximum(d_buffers.state.time_end.abs(), d_buffers.state.actual_time.abs()).max().item()
-    phmax = cp[2 * nk:3 * nk].abs().max().item()
+    # one device-to-host read for the three maxima
+    w, tm, ph = torch.stack([(cp[nk:2 * nk].abs() * sp[1:2].abs()).max(),
+                             torch.maximum(d_buffers.state.time_end.abs(), d_buffers.state.actual_time.abs()).max(),
+                             cp[2 * nk:3 * nk].abs().max()]).tolist()
+    wmax, tmax, phmax = w * 2.0 * np.pi, tm, ph
     if wmax * tmax + phmax >= _MAX_TRIG_ARG:
         raise ValueError("time horizon too long for the device sincos range "
                          f"(|theta| up to {wmax * tmax + phmax:.3g} >= {_MAX_TRIG_ARG:.3g})")

@@ -3,12 +3,13 @@
 //
 // The reference runs set_action, observe, check_termination, get_rewards,
 // advance_time, final_termination and reset_envs as some forty torch kernels
-// and three host synchronisations per step.  Here they are three kernels, one
+// and three host synchronisations per step.  Here they are four kernels, one
 // thread per environment:
 //
 //   env_set_action  actions -> control_params rows                     (:732-749)
 //   env_post        observe + check_termination + get_rewards
-//                   + advance_time + the ordered done list               (:756-842)
+//                   + advance_time + done counts and offsets             (:756-842)
+//   env_compact     the ordered done list from the per-workgroup ids
 //   env_reset       final outputs of the done envs + reset_envs with the
 //                   scene arithmetic of SceneGenerator                   (:112-249, 583-614)
 //
@@ -37,11 +38,11 @@ namespace {
 
 constexpr int kBlock = 256;                   // threads per workgroup (4 waves)
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxEnvs = 1 << 20;             // 4096 workgroups: the done-list scan fits in LDS
-constexpr int kMaxBlocks = kMaxEnvs / kBlock;
+constexpr int kMaxEnvs = 1 << 20;             // 4096 workgroups: 16 per scanning thread
 constexpr int kMaxStack = 64;
 // workspace words: [0] ticket, [1..4] counts (done, failures, successes,
-// time-outs), [8 .. 8 + 4 nb) per-workgroup counts, then NE int32 local ids
+// time-outs), [8 .. 8 + 4 nb) per-workgroup counts, [8 + 4 nb .. 8 + 5 nb)
+// per-workgroup offsets of the done list, then nb * kBlock int32 local ids
 constexpr int kWsCounts = 1;
 constexpr int kWsBlocks = 8;
 
@@ -247,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
         b.time_outs[e] = f_tout;
     }
 
-    // final_termination's done list (:832): local ids per workgroup, then the
-    // last workgroup to finish concatenates them in workgroup order
+    // final_termination's done list (:832): local ids per workgroup; the last
+    // workgroup to finish turns the counts into offsets
     const bool done = live && (f_fail || f_succ || f_tout);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t m_done = __ballot(done);
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
     __syncthreads();
     uint32_t* ws = b.workspace;
     const int nb = gridDim.x;
-    int32_t* local = reinterpret_cast<int32_t*>(ws + kWsBlocks + 4 * nb);
+    int32_t* local = reinterpret_cast<int32_t*>(ws + kWsBlocks + 5 * nb);
     if (done) {
         uint32_t before = 0;
         for (int w = 0; w < wave; ++w) before += wcnt[0][w];
@@ -285,18 +286,16 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
     if (!last) return;
     __threadfence();
 
-    // the last workgroup: exclusive scan of the per-workgroup done counts
-    __shared__ uint32_t off[kMaxBlocks];
+    // the last workgroup: exclusive scan of the per-workgroup done counts into
+    // per-workgroup offsets (env_compact then moves every workgroup's ids)
     __shared__ uint32_t part[kBlock];
     __shared__ uint32_t tot[4];
     const int q = (nb + kBlock - 1) / kBlock;
     const int b0 = threadIdx.x * q;
     const int b1 = min(nb, b0 + q);
     uint32_t sums[4] = {0u, 0u, 0u, 0u};
-    for (int bb = b0; bb < b1; ++bb) {
-        off[bb] = sums[0];
+    for (int bb = b0; bb < b1; ++bb)
         for (int j = 0; j < 4; ++j) sums[j] += ws[kWsBlocks + 4 * bb + j];
-    }
     part[threadIdx.x] = sums[0];
     if (threadIdx.x < 4) tot[threadIdx.x] = 0u;
     __syncthreads();
@@ -308,13 +307,11 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
         part[threadIdx.x] += v;
         __syncthreads();
     }
-    const uint32_t excl = part[threadIdx.x] - sums[0];
-    for (int bb = b0; bb < b1; ++bb) off[bb] += excl;
-    __syncthreads();
-    for (int bb = 0; bb < nb; ++bb) {
-        const uint32_t c = ws[kWsBlocks + 4 * bb];
-        if ((uint32_t)threadIdx.x < c)
-            b.done_ids[off[bb] + threadIdx.x] = (int64_t)local[(size_t)bb * kBlock + threadIdx.x];
+    uint32_t run = part[threadIdx.x] - sums[0];
+    uint32_t* offs = ws + kWsBlocks + 4 * nb;
+    for (int bb = b0; bb < b1; ++bb) {
+        offs[bb] = run;
+        run += ws[kWsBlocks + 4 * bb];
     }
     if (threadIdx.x == 0) {
         ws[kWsCounts + 0] = part[kBlock - 1];
@@ -323,6 +320,19 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
         ws[kWsCounts + 3] = tot[3];
         ws[0] = 0u;   // ticket for the next launch (stream-ordered after this one)
     }
+}
+
+// Second half of the done list: workgroup b of env_post wrote its ids in
+// ascending order at local[b * kBlock ..); they move to done_ids at the
+// workgroup's offset.  One workgroup per env_post workgroup, no serial chain.
+__global__ __launch_bounds__(kBlock) void env_compact(const uint32_t* ws, int nb, int64_t* done_ids)
+{
+    const int bb = blockIdx.x;
+    const uint32_t c = ws[kWsBlocks + 4 * bb];
+    if ((uint32_t)threadIdx.x >= c) return;
+    const uint32_t o = ws[kWsBlocks + 4 * nb + bb];
+    const int32_t* local = reinterpret_cast<const int32_t*>(ws + kWsBlocks + 5 * nb);
+    done_ids[o + threadIdx.x] = (int64_t)local[(size_t)bb * kBlock + threadIdx.x];
 }
 
 __global__ __launch_bounds__(kBlock) void env_reset(ResetArgs a)
@@ -465,7 +475,7 @@ extern "C" {
 int64_t cbc_env_workspace_words(int32_t num_envs)
 {
     if (num_envs < 1) return 0;
-    return (int64_t)kWsBlocks + 4 * (int64_t)n_blocks(num_envs) + (int64_t)n_blocks(num_envs) * kBlock;
+    return (int64_t)kWsBlocks + 5 * (int64_t)n_blocks(num_envs) + (int64_t)n_blocks(num_envs) * kBlock;
 }
 
 int32_t cbc_env_set_action(const cbc_env_spec* spec, const cbc_env_buffers* buf, void* stream)
@@ -499,6 +509,14 @@ int32_t cbc_env_post(const cbc_env_spec* spec, const cbc_env_buffers* buf, int64
     hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_post), dim3(n_blocks(spec->num_envs)),
                                    dim3(kBlock), args, 0, st);
     if (e != hipSuccess) return hip_fail(e, "env_post launch");
+    {
+        const uint32_t* ws = buf->workspace;
+        int nb = n_blocks(spec->num_envs);
+        int64_t* ids = buf->done_ids;
+        void* cargs[] = {&ws, &nb, &ids};
+        e = hipLaunchKernel(reinterpret_cast<const void*>(&env_compact), dim3(nb), dim3(kBlock), cargs, 0, st);
+        if (e != hipSuccess) return hip_fail(e, "env_compact launch");
+    }
     if (!counts) return CBC_OK;
     uint32_t h[4];
     e = hipMemcpyAsync(h, buf->workspace + kWsCounts, sizeof(h), hipMemcpyDeviceToHost, st);

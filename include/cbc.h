@@ -415,8 +415,9 @@ int32_t cbc_env_set_action(const cbc_env_spec *spec, const cbc_env_buffers *buf,
 
 /* observe (:756-769), check_termination (:772-789), get_rewards (:792-813),
  * advance_time (:816-827) and the done list of final_termination (:830-842),
- * one thread per env, in one launch; the last workgroup to finish writes the
- * ordered done_ids and the counts.  When `counts` is not NULL the call waits
+ * one thread per env, in one launch; its last workgroup to finish turns the
+ * per-workgroup done counts into offsets, and a second launch moves each
+ * workgroup's ids into the ordered done_ids.  When `counts` is not NULL the call waits
  * for the stream and returns them (the reference's host syncs of the same
  * step); otherwise it is asynchronous. */
 int32_t cbc_env_post(const cbc_env_spec *spec, const cbc_env_buffers *buf, int64_t observed, void *stream,
