@@ -499,7 +499,7 @@ def bench_env(args):
     n, ne = args.env_bubbles, args.env_count
     steps = args.steps or 20
     log(f"env: building PosNBC1D {ne} envs x {n} bubbles")
-    env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False)
+    env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False, collect_trajectories=False)
     T = float(env.time_step_length)
     env.reset()
     stream = torch.cuda.current_stream()

@@ -145,6 +145,22 @@ class EnvCountsC(ctypes.Structure):
     _fields_ = [("done", _i32), ("failures", _i32), ("successes", _i32), ("time_outs", _i32)]
 
 
+class TrajSpecC(ctypes.Structure):
+    _fields_ = [("num_envs", _i32), ("num_bubbles", _i32), ("tile", _i32), ("ndo", _i32), ("num_saved", _i32),
+                ("saved_rows", _i32 * 4), ("obs_dim", _i32), ("num_actions", _i32), ("max_steps", _i32),
+                ("max_points", _i64), ("step_length", _f64)]
+
+
+class TrajBuffersC(ctypes.Structure):
+    _fields_ = [(name, _vp) for name in (
+        "observations", "actions", "rewards", "dense_index", "dense_time", "dense_state",
+        "obs", "act", "rew", "time", "states", "steps", "points", "overflow")]
+
+
+class TrajPackedC(ctypes.Structure):
+    _fields_ = [("obs", _vp), ("act", _vp), ("rew", _vp), ("time", _vp), ("states", _vp), ("total_points", _i64)]
+
+
 # name -> (restype, argtypes); every entry point of include/cbc.h
 SIGNATURES = {
     "cbc_abi_version": (_i32, []),
@@ -176,6 +192,9 @@ SIGNATURES = {
     "cbc_env_reset": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), ctypes.POINTER(EnvSceneC),
                              _vp, _i32, _i64, _vp, _vp, _vp, _vp]),
     "cbc_env_scene_positions": (_i32, [ctypes.POINTER(EnvSceneC), _i32, _i32, _vp, _vp, _vp]),
+    "cbc_traj_append": (_i32, [ctypes.POINTER(TrajSpecC), ctypes.POINTER(TrajBuffersC), _vp]),
+    "cbc_traj_gather": (_i32, [ctypes.POINTER(TrajSpecC), ctypes.POINTER(TrajBuffersC), _vp, _i32, _vp, _vp,
+                               ctypes.POINTER(TrajPackedC), _vp]),
 }
 
 

@@ -25,7 +25,11 @@ Differences (all deliberate):
   reference writes the host buffers, which reach the device only when the
   device buffers are first created);
 * the reference's default backend (Numba) is replaced by HIP; ``render_env``
-  and trajectory collection are not built here.
+  is not built;
+* with ``collect_trajectories`` the episodes are recorded on the device and
+  written as the reference's zarr files (envs/common/trajectory_buffer.py);
+  the dense-output buffers are not zeroed after each step (only the point
+  counts, which is all the solver and the recorder read).
 """
 from __future__ import annotations
 
@@ -40,6 +44,7 @@ from ..backends.hip.env_step import EnvKernels, ObsPart, scene_positions
 from ..models import CKM1D
 from .bubble_vec_env import BubbleVecEnv
 from .common.specs import ActionComponent, ActionMapingManager, ObservationComponent, SpaceSchema
+from .common.trajectory_buffer import TrajectorBuffer
 
 
 class SceneDraws:
@@ -256,7 +261,7 @@ class PosNBC1D(BubbleVecEnv):
             positive_terminal_reward: float = 10.0,
             negative_terminal_reward: float = -10.0,
             render_env: bool = False,
-            collect_trajectories: bool = False,
+            collect_trajectories: bool = True,
             trajectory_resolution: int = 128,
             trajectory_buffer_store: str = "PosNBC1D",
             backend: BackendName = BackendName.HIP,
@@ -277,8 +282,6 @@ class PosNBC1D(BubbleVecEnv):
         reference does."""
         if render_env:
             raise NotImplementedError("render_env: rendering is not part of this package")
-        if collect_trajectories:
-            raise NotImplementedError("collect_trajectories: trajectory collection is not built yet")
         self.action_space_schema = action_space_schema if action_space_schema is not None else _default_action_schema()
         self.observation_space_schema = (observation_space_schema if observation_space_schema is not None
                                          else _default_observation_schema(num_bubbles))
@@ -337,7 +340,26 @@ class PosNBC1D(BubbleVecEnv):
         self.apply_termination = apply_termination
         self.position_tolerance = position_tolerance
         self._collect_trajectories = collect_trajectories
-        self.trajectory_buffer = None
+        self.trajectory_buffer = TrajectorBuffer(
+            num_envs=self.num_envs, num_units=self.num_bubbles, saved_states=[0, 1],
+            zarr_root=trajectory_buffer_store, max_episode_steps=episode_length, verbose=verbose,
+            metadata={
+                "num_bubbles": self.num_bubbles,
+                "ac_type": acoustic_field,
+                "k": num_components,
+                "PA_max": self.action_space_schema.components[0].max,
+                "PA_min": self.action_space_schema.components[0].min,
+                "PS_max": self.action_space_schema.components[1].max,
+                "PS_min": self.action_space_schema.components[1].max,    # as the reference stores it
+                "freq": self.default_FR,
+                "ref_freq": self.REF_FREQ,
+                "XT_max": self.observation_space_schema.components[0].max,
+                "XT_min": self.observation_space_schema.components[0].min,
+                "X_max": self.observation_space_schema.components[1].max,
+                "X_min": self.observation_space_schema.components[1].max,    # as the reference stores it
+                "position_tolerance": self.position_tolerance,
+                "time_step_length": self.time_step_length,
+            }) if collect_trajectories else None
 
         parts = []
         for c in comps:
@@ -407,6 +429,10 @@ class PosNBC1D(BubbleVecEnv):
         self.set_action(action)
         self.step_physics()
         self._post_step()
+        if self.trajectory_buffer is not None:
+            dd = self.model.runtime.device_buffers.dense
+            self.trajectory_buffer.step(self._observations, self._actions, self._rewards, dd.dense_index,
+                                        dd.dense_time, dd.dense_state)
         info = self.final_termination()
         out = (self._observations, self._rewards, self._terminateds, self._time_outs, info)
         if clone:
@@ -447,6 +473,8 @@ class PosNBC1D(BubbleVecEnv):
         ids = self._kernels.done_ids[:k].clone()
         d = self.scene_generator.draw(k, torch.float64, self.device, self._kernels.device)
         fo, fr, fl = self._kernels.reset(d.scene, ids, self._observed, final=True)
+        if self.trajectory_buffer is not None:
+            self.trajectory_buffer.end_episode(ids, fo, fl, fr, np.tile(self.default_R0, (k, 1)))
         return {"final_observation": fo, "dones": ids, "episode_return": fr, "episode_length": fl}
 
     # -- resets (:541-614) -----------------------------------------------------

@@ -440,6 +440,70 @@ int32_t cbc_env_reset(const cbc_env_spec *spec, const cbc_env_buffers *buf, cons
 int32_t cbc_env_scene_positions(const cbc_env_scene *scene, int32_t n, int32_t num_bubbles, double *positions,
                                 double *targets, void *stream);
 
+/* ---- trajectory collection -----------------------------------------------
+ * TrajectorBuffer (coupledbubble_control/envs/common/trajectory_buffer.py:70-166)
+ * on the device: every env's running episode is kept in an arena of float32
+ * rows (the dtype TrajectoryZarrWriter stores, :196-226):
+ *   obs (NE, max_steps, obs_dim), act (NE, max_steps, num_actions),
+ *   rew (NE, max_steps), time (NE, max_points),
+ *   states (NE, num_saved, max_points, N), steps (NE,) int32, points (NE,) int64.
+ * Points of step k > 0 skip the step's first dense point and carry times
+ * t + k * step_length (Trajectory.merge_list, :38-56). */
+typedef struct cbc_traj_spec {
+    int32_t num_envs;
+    int32_t num_bubbles;
+    int32_t tile;
+    int32_t ndo;                 /* dense points per solve (NDO) */
+    int32_t num_saved;           /* state rows recorded (saved_states) */
+    int32_t saved_rows[4];
+    int32_t obs_dim;
+    int32_t num_actions;
+    int32_t max_steps;           /* arena rows per env (episode_length) */
+    int64_t max_points;          /* arena points per env */
+    double step_length;          /* time_step_length */
+} cbc_traj_spec;
+
+typedef struct cbc_traj_buffers {
+    /* one step's inputs (device) */
+    const float *observations;   /* (NE, obs_dim) */
+    const float *actions;        /* (NE, num_actions) */
+    const float *rewards;        /* (NE,) */
+    int32_t *dense_index;        /* the solver's dense buffers; dense_index is cleared */
+    const double *dense_time;
+    const double *dense_state;
+    /* the arena */
+    float *obs;
+    float *act;
+    float *rew;
+    float *time;
+    float *states;
+    int32_t *steps;
+    int64_t *points;
+    int32_t *overflow;           /* set to 1 when an append did not fit (the step is not recorded) */
+} cbc_traj_buffers;
+
+/* Packed trajectories of n envs, back to back in done-list order: obs / act /
+ * rew rows (sum of steps), time (total_points), states (num_saved, total_points, N). */
+typedef struct cbc_traj_packed {
+    float *obs;
+    float *act;
+    float *rew;
+    float *time;
+    float *states;
+    int64_t total_points;
+} cbc_traj_packed;
+
+/* TrajectorBuffer.step (:95-127) for every env; one wave per env.  Asynchronous. */
+int32_t cbc_traj_append(const cbc_traj_spec *spec, const cbc_traj_buffers *buf, void *stream);
+
+/* The arenas of envs ids[0..n) into `out` (step_off / point_off: device int64
+ * prefix offsets of each env's rows and points, from the steps / points
+ * counters), then their counters cleared -- the data TrajectorBuffer.end_episode
+ * (:129-155) merges and writes.  One workgroup per env.  Asynchronous. */
+int32_t cbc_traj_gather(const cbc_traj_spec *spec, const cbc_traj_buffers *buf, const int64_t *ids, int32_t n,
+                        const int64_t *step_off, const int64_t *point_off, const cbc_traj_packed *out,
+                        void *stream);
+
 /* ---- diagnostics ---------------------------------------------------------
  * Evaluates the kernels' elementary functions on n device inputs (x, and y for
  * two-argument functions) into out[2 n] (value pairs (u, v) per input):

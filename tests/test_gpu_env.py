@@ -273,7 +273,7 @@ def test_env_episodes_match_oracle(device, n, alignment):
     ne = 64
     env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, R0=[40.0, 60.0, 50.0][:n], PA=[1.0, 0.5],
                    time_step_length=2.0, episode_length=3, position_tolerance=0.05, alignment=alignment,
-                   seed=17, verbose=False)
+                   seed=17, verbose=False, collect_trajectories=False)
     obs, info = env.reset()
     assert info == {}
     O = _oracle_env(env)
@@ -324,7 +324,7 @@ def test_env_api_behaviour(device, capsys):
 
     def make(**kw):
         return PosNBC1D(num_bubbles=2, num_envs=128, envs_per_block=64, time_step_length=1.0, episode_length=2,
-                        seed=5, **kw)
+                        seed=5, collect_trajectories=False, **kw)
 
     a, b = make(verbose=False), make(verbose=True)
     oa, _ = a.reset()

@@ -216,3 +216,94 @@ def test_env_struct_layout_matches_header(tmp_path):
     for line in out[1:]:
         c, f, off = line.split()
         assert getattr(structs[c], f).offset == int(off), (c, f)
+
+
+def test_trajectory_recorder_known_answer():
+    """Two steps of one env: step 1 keeps all its dense points, step 2 drops its
+    first and is offset by 5 (the Trajectory default); the final observation
+    is appended."""
+    from oracle.trajectory import TrajectoryRecorder
+
+    rec = TrajectoryRecorder(1, 2, saved_states=(0, 1))
+    dense_time = np.zeros((4, 1))
+    dense_state = np.zeros((4, 4, 1, 2))
+    for k, (times, xs) in enumerate((([0.0, 0.5, 1.0], [1.0, 2.0, 3.0]), ([0.0, 0.25, 1.0], [3.0, 4.0, 5.0]))):
+        n = len(times)
+        dense_time[:n, 0] = times
+        dense_state[:n, 0, 0, :] = np.array(xs)[:, None]
+        dense_state[:n, 1, 0, :] = -np.array(xs)[:, None]
+        idx = np.array([n], dtype=np.int32)
+        rec.step(np.array([[k, k]], dtype=F32), np.array([[0.5 * k]], dtype=F32), np.array([k], dtype=F32), idx,
+                 dense_time, dense_state)
+        assert idx[0] == 0
+    rec.end_episode([0], np.array([[9.0, 9.0]]), [2], [1.5], [[60.0, 40.0]])
+    ep = rec.episodes[0]
+    assert ep["observations"].tolist() == [[0, 0], [1, 1], [9, 9]]
+    assert ep["actions"].tolist() == [[0.0], [0.5]] and ep["rewards"].tolist() == [[0.0], [1.0]]
+    assert ep["dense_time"].tolist() == [0.0, 0.5, 1.0, 5.25, 6.0]
+    assert ep["dense_state_0"][:, 0].tolist() == [1.0, 2.0, 3.0, 4.0, 5.0]
+    assert ep["dense_state_1"][:, 1].tolist() == [-1.0, -2.0, -3.0, -4.0, -5.0]
+    assert ep["episode_length"] == 2 and ep["equilibrium_radii"].tolist() == [60.0, 40.0]
+
+
+def test_zarr_v3_store_layout_and_round_trip(tmp_path):
+    """Stores follow the Zarr v3 core spec (zarr.json per node, regular grid,
+    default chunk keys c/0/..., bytes codec); arrays and attributes round-trip."""
+    import json
+
+    from coupledbubble_control.envs.common import zarr_v3 as Z
+    from coupledbubble_control.envs.common.trajectory_buffer import Trajectory, TrajectoryZarrWriter
+
+    w = TrajectoryZarrWriter(str(tmp_path / "store"), shard_size=1000, global_metadata={"k": 2, "freq": [25.0, 50.0]},
+                             verbose=False)
+    meta = json.loads((tmp_path / "store" / "global_metadata.zarr" / "zarr.json").read_text())
+    assert meta == {"zarr_format": 3, "node_type": "group", "attributes": {"k": 2, "freq": [25.0, 50.0],
+                                                                            "_initialized": True}}
+    rng = np.random.default_rng(0)
+    trj = Trajectory(env_id=3, num_units=2)
+    trj.observations = [rng.random((4, 6)).astype(F32)]
+    trj.actions = [rng.random((3, 4)).astype(F32)]
+    trj.rewards = [rng.random((3, 1)).astype(F32)]
+    trj.dense_time = [np.linspace(0, 1, 7)]
+    trj.dense_states[0] = [rng.random((7, 2))]
+    trj.episode_length, trj.episode_reward = np.int64(3), np.float32(-1.25)
+    for eid in (999, 1000, 1000):
+        w.write(eid, trj)
+    g = tmp_path / "store" / "shard_01000_01999.zarr" / "trj_001000"
+    assert (tmp_path / "store" / "shard_00000_00999.zarr" / "trj_000999" / "zarr.json").is_file()
+    doc = json.loads((g / "observations" / "zarr.json").read_text())
+    assert doc["zarr_format"] == 3 and doc["node_type"] == "array" and doc["shape"] == [4, 6]
+    assert doc["data_type"] == "float32" and doc["fill_value"] == 0.0
+    assert doc["chunk_grid"] == {"name": "regular", "configuration": {"chunk_shape": [4, 6]}}
+    assert doc["chunk_key_encoding"] == {"name": "default", "configuration": {"separator": "/"}}
+    assert doc["codecs"] == [{"name": "bytes", "configuration": {"endian": "little"}}]
+    raw = (g / "observations" / "c" / "0" / "0").read_bytes()
+    assert raw == trj.observations[0].astype("<f4").tobytes()
+    assert np.array_equal(Z.read_array(g / "observations"), trj.observations[0])
+    assert np.array_equal(Z.read_array(g / "dense_time"), np.linspace(0, 1, 7).astype(F32))
+    assert np.array_equal(Z.read_array(g / "dense_state_0"), trj.dense_states[0][0].astype(F32))
+    assert Z.read_array(g / "equilibrium_radii").tolist() == [0.0, 0.0]
+    attrs = Z.group_attrs(g)
+    assert attrs["env_id"] == 1000 and attrs["episode_length"] == 3 and attrs["episode_reward"] == -1.25
+    assert Z.children(g.parent) == ["trj_001000"]
+    e = Z.write_array(tmp_path, "empty", np.zeros((0, 3)))
+    assert json.loads((e / "zarr.json").read_text())["chunk_grid"]["configuration"]["chunk_shape"] == [1, 3]
+    assert not (e / "c").exists() and Z.read_array(e).shape == (0, 3)
+    # metadata is written once: a second writer does not overwrite it
+    TrajectoryZarrWriter(str(tmp_path / "store"), global_metadata={"k": 5}, verbose=False)
+    assert Z.group_attrs(tmp_path / "store" / "global_metadata.zarr")["k"] == 2
+
+
+def test_product_box_scaling_equals_oracle():
+    """The observation space built from a schema (product specs/spaces) scales
+    with the same float32 bias and reciprocal as the oracle's box_scale."""
+    from coupledbubble_control.envs.common.specs import ObservationComponent, SpaceSchema
+
+    sch = SpaceSchema()
+    sch.components = [ObservationComponent("XT", [0, 1, 2], [-0.5, -1.0, -0.3], [0.5, 1.0, 0.7]),
+                      ObservationComponent("X", [0, 1, 2], [-0.5, -1.0, -0.3], [0.5, 1.0, 0.7], stack=3)]
+    box = sch.build_space(4, "cpu", seed=1)
+    bias, rscale = E.box_scale([E.Part(c.name, c.min, c.max, c.stack) for c in sch.components])
+    assert box._bias.numpy().tobytes() == bias.tobytes()
+    assert box._rscale.numpy().tobytes() == rscale.tobytes()
+    assert box.space_data.shape == (12,)
