@@ -90,6 +90,10 @@ def parse():
     p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
     p.add_argument("--env-bubbles", type=int, default=2, help="env: bubbles per env")
     p.add_argument("--env-count", type=int, default=16384, help="env: number of envs")
+    p.add_argument("--env-trajectories", type=int, default=0,
+                   help="env: record trajectories (dense output, device arena, zarr files in a temporary "
+                        "directory); 2 writes the files on a background thread")
+    p.add_argument("--env-episode-length", type=int, default=128, help="env: steps per episode")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample length")
     return p.parse_args()
@@ -498,12 +502,20 @@ def bench_env(args):
 
     n, ne = args.env_bubbles, args.env_count
     steps = args.steps or 20
-    log(f"env: building PosNBC1D {ne} envs x {n} bubbles")
-    env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False, collect_trajectories=False)
+    import shutil
+    import tempfile
+
+    traj = bool(args.env_trajectories)
+    store = tempfile.mkdtemp(prefix="cbc_traj_") if traj else None
+    log(f"env: building PosNBC1D {ne} envs x {n} bubbles" + (f", trajectories into {store}" if traj else ""))
+    env = PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False, collect_trajectories=traj,
+                   trajectory_buffer_store=store or "PosNBC1D", episode_length=args.env_episode_length,
+                   trajectory_async_write=args.env_trajectories == 2)
     T = float(env.time_step_length)
     env.reset()
     stream = torch.cuda.current_stream()
-    ev = {"physics": [], "post": [], "reset": []}
+    ev = {"physics": [], "post": [], "reset": [], "append": []}
+    write_s = []
 
     def timed(fn, key):
         def run(*a, **kw):
@@ -518,12 +530,25 @@ def bench_env(args):
     env.step_physics = timed(env.step_physics, "physics")
     env._kernels.post = timed(env._kernels.post, "post")
     env._kernels.reset = timed(env._kernels.reset, "reset")
+    if traj:
+        tb = env.trajectory_buffer
+        tb.step = timed(tb.step, "append")
+        end_episode = tb.end_episode
+
+        def timed_end(*a, **kw):
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            end_episode(*a, **kw)
+            write_s.append(time.perf_counter() - c0)
+        tb.end_episode = timed_end
     log(f"env: {args.warmup} warmup steps")
     for _ in range(args.warmup):
         env.step(None)
     torch.cuda.synchronize()
     for v in ev.values():
         v.clear()
+    write_s.clear()
+    written0 = env.trajectory_buffer.episode_count if traj else 0
     a0 = attempts_of([env.model])[n]
     lat, dones = [], 0
     log(f"env: {steps} timed steps")
@@ -536,11 +561,16 @@ def bench_env(args):
         dones += int(info["dones"].numel()) if info else 0
     wall = time.perf_counter() - t0
     att = attempts_of([env.model])[n] - a0
+    flush_s = 0.0
+    if traj:
+        c0 = time.perf_counter()
+        env.trajectory_buffer.flush()
+        flush_s = time.perf_counter() - c0
 
     def ms(key):
         return [e0.elapsed_time(e1) for e0, e1 in ev[key]]
 
-    phys, post, rst = ms("physics"), ms("post"), ms("reset")
+    phys, post, rst, app = ms("physics"), ms("post"), ms("reset"), ms("append")
     lat_ms = np.array(lat) * 1e3
     flops = float(att) * flops_per_attempt(n)
     out = {
@@ -554,9 +584,20 @@ def bench_env(args):
                               "host_rest": float(np.mean(lat_ms) - np.mean(phys) - np.mean(post)
                                                  - (np.sum(rst) / steps if rst else 0.0))},
         "episodes_finished": dones,
+        "episode_length": args.env_episode_length,
         "roofline": roofline(flops, float(np.sum(phys)) / 1e3, window="solver launches of the timed steps",
                              attempts={str(n): att}),
     }
+    if traj:
+        written = env.trajectory_buffer.episode_count - written0
+        nfiles = sum(len(f) for _, _, f in os.walk(store))
+        out["trajectories"] = {
+            "append_ms_per_step": float(np.mean(app)), "episodes_written": written,
+            "end_episode_s_total": float(np.sum(write_s)),
+            "episodes_per_s_written": written / float(np.sum(write_s)) if write_s else None,
+            "files": nfiles, "dense_points_per_solve": int(env.model._execution_spec.ndo),
+            "background_writer": args.env_trajectories == 2, "flush_after_timed_steps_s": flush_s}
+        shutil.rmtree(store, ignore_errors=True)
     if args.cpu_baseline:
         log("env: CPU baseline (oracle environment)")
         out["cpu_baseline"] = _env_oracle_sample(n, T, args.cpu_seconds)

@@ -275,11 +275,13 @@ class PosNBC1D(BubbleVecEnv):
             device: str = "cuda",
             keep_step: bool = False,
             verbose: bool = True,
+            trajectory_async_write: bool = False,
     ) -> None:
         """Arguments as the reference's (:260-354).  Extra: ``keep_step`` keeps each
         env's last accepted step size across steps instead of resetting it to 1e-6;
         ``verbose`` prints the per-step failure / success / time-out counts as the
-        reference does."""
+        reference does; ``trajectory_async_write`` writes episode files on a
+        background thread (``close()`` waits for it)."""
         if render_env:
             raise NotImplementedError("render_env: rendering is not part of this package")
         self.action_space_schema = action_space_schema if action_space_schema is not None else _default_action_schema()
@@ -343,6 +345,7 @@ class PosNBC1D(BubbleVecEnv):
         self.trajectory_buffer = TrajectorBuffer(
             num_envs=self.num_envs, num_units=self.num_bubbles, saved_states=[0, 1],
             zarr_root=trajectory_buffer_store, max_episode_steps=episode_length, verbose=verbose,
+            async_write=trajectory_async_write,
             metadata={
                 "num_bubbles": self.num_bubbles,
                 "ac_type": acoustic_field,
@@ -511,6 +514,8 @@ class PosNBC1D(BubbleVecEnv):
 
     @torch.no_grad()
     def close(self):
+        if self.trajectory_buffer is not None:
+            self.trajectory_buffer.close()
         print("close")
 
     def render(self):

@@ -23,6 +23,7 @@ environment leaves in place).
 from __future__ import annotations
 
 import ctypes
+import os
 import time
 from collections import defaultdict
 from dataclasses import dataclass, field
@@ -79,7 +80,9 @@ class TrajectoryZarrWriter:
     """One zarr group per episode in shards of ``shard_size`` episodes (reference :158-226)."""
 
     def __init__(self, root_dir: str, shard_size: int = 1000, global_metadata: Optional[Dict] = None,
-                 verbose: bool = True) -> None:
+                 verbose: bool = True, async_write: bool = False) -> None:
+        """``async_write`` writes the episodes in order on one background thread, so
+        file output overlaps the next steps; ``flush()`` / ``close()`` wait for it."""
         self.root_dir = Path(root_dir)
         self.root_dir.mkdir(parents=True, exist_ok=True)
         if verbose:
@@ -87,6 +90,16 @@ class TrajectoryZarrWriter:
         self.shard_size = shard_size
         self.global_metadata: Dict = global_metadata if global_metadata is not None else {}
         self._write_global_metadata()
+        self._shards = set()
+        self._queue = self._thread = None
+        self._error: Optional[BaseException] = None
+        if async_write:
+            import queue
+            import threading
+
+            self._queue = queue.Queue()
+            self._thread = threading.Thread(target=self._run, name="trajectory-writer", daemon=True)
+            self._thread.start()
 
     def _write_global_metadata(self):
         if not self.global_metadata:
@@ -101,28 +114,65 @@ class TrajectoryZarrWriter:
         start = (episode_id // self.shard_size) * self.shard_size
         return self.root_dir / f"shard_{start:05d}_{start + self.shard_size - 1:05d}.zarr"
 
-    def _create_group(self, episode_id: int):
+    def _group_path(self, episode_id: int) -> Path:
+        """The episode's group path in its shard (the shard created when missing, an
+        existing group of the same name removed, as the reference does)."""
         shard = self._shard_path(episode_id)
-        zarr_v3.open_group(shard)
-        name = f"trj_{episode_id:06d}"
-        if zarr_v3.is_node(shard / name):
-            zarr_v3.delete(shard / name)
-        return zarr_v3.create_group(shard, name), shard
+        if shard not in self._shards:
+            zarr_v3.open_group(shard)
+            self._shards.add(shard)
+        g = shard / f"trj_{episode_id:06d}"
+        if os.path.lexists(g):
+            zarr_v3.delete(g)
+        return g
+
+    def _write(self, episode_id: int, trj: Trajectory) -> None:
+        data = trj.merge_list()
+        arrays = {"observations": data["observations"], "actions": data["actions"], "rewards": data["rewards"]}
+        if trj.equilibrium_radii is not None:
+            arrays["equilibrium_radii"] = trj.equilibrium_radii
+        arrays["dense_time"] = data["dense_time"]
+        for idx, arr in data["dense_state"].items():
+            arrays[f"dense_state_{idx:0d}"] = arr
+        zarr_v3.write_new_group(self._group_path(episode_id), arrays, dict(
+            env_id=int(episode_id), num_units=int(trj.num_units), episode_length=trj.episode_length,
+            episode_reward=trj.episode_reward, saved_unix=int(time.time())))
 
     def write(self, episode_id: int, trj: Trajectory) -> None:
-        group, _ = self._create_group(episode_id)
-        data = trj.merge_list()
-        zarr_v3.write_array(group, "observations", data["observations"])
-        zarr_v3.write_array(group, "actions", data["actions"])
-        zarr_v3.write_array(group, "rewards", data["rewards"])
-        if trj.equilibrium_radii is not None:
-            zarr_v3.write_array(group, "equilibrium_radii", trj.equilibrium_radii)
-        zarr_v3.write_array(group, "dense_time", data["dense_time"])
-        for idx, arr in data["dense_state"].items():
-            zarr_v3.write_array(group, f"dense_state_{idx:0d}", arr)
-        zarr_v3.update_attrs(group, dict(env_id=int(episode_id), num_units=int(trj.num_units),
-                                         episode_length=trj.episode_length, episode_reward=trj.episode_reward,
-                                         saved_unix=int(time.time())))
+        """One episode's group (reference :196-226); queued when writing in the background."""
+        if self._queue is not None:
+            if self._error is not None:
+                raise RuntimeError("trajectory writer thread failed") from self._error
+            self._queue.put((episode_id, trj))
+            return
+        self._write(episode_id, trj)
+
+    def flush(self) -> None:
+        """Wait until every queued episode is on disk (background writing only)."""
+        if self._queue is not None:
+            self._queue.join()
+            if self._error is not None:
+                raise RuntimeError("trajectory writer thread failed") from self._error
+
+    def close(self) -> None:
+        if self._queue is not None:
+            self.flush()
+            self._queue.put(None)
+            self._thread.join()
+            self._queue = None
+
+    def _run(self) -> None:
+        while True:
+            item = self._queue.get()
+            try:
+                if item is None:
+                    return
+                if self._error is None:
+                    self._write(*item)
+            except BaseException as e:  # reported to the caller on its next write / flush
+                self._error = e
+            finally:
+                self._queue.task_done()
 
 
 class TrajectorBuffer:
@@ -134,7 +184,8 @@ class TrajectorBuffer:
 
     def __init__(self, num_envs: int, num_units: int, saved_states: Optional[Sequence[int]] = None, *,
                  zarr_root: str, zarr_shard_size: int = 1000, metadata: Optional[Dict] = None,
-                 max_episode_steps: int = 128, time_step_length: float = 5.0, verbose: bool = True) -> None:
+                 max_episode_steps: int = 128, time_step_length: float = 5.0, verbose: bool = True,
+                 async_write: bool = False) -> None:
         self.episode_count = 0
         self._num_envs = num_envs
         self._num_units = num_units
@@ -142,8 +193,16 @@ class TrajectorBuffer:
         self._max_steps = int(max_episode_steps)
         self._step_length = float(time_step_length)
         self._verbose = verbose
-        self._zarr_writer = TrajectoryZarrWriter(zarr_root, zarr_shard_size, metadata, verbose=verbose)
+        self._zarr_writer = TrajectoryZarrWriter(zarr_root, zarr_shard_size, metadata, verbose=verbose,
+                                                 async_write=async_write)
         self._arena = None
+
+    def flush(self) -> None:
+        """Wait for the background writer (no-op for synchronous writing)."""
+        self._zarr_writer.flush()
+
+    def close(self) -> None:
+        self._zarr_writer.close()
 
     # -- the device arena ------------------------------------------------------
     def _allocate(self, observations, actions, dense_state):
@@ -248,10 +307,11 @@ class TrajectorBuffer:
         ep_len = torch.as_tensor(episode_lengths).cpu().numpy()
         ep_rew = torch.as_tensor(episode_rewards).cpu().numpy()
         eq = np.asanyarray(equilibrium_radii, dtype=np.float32)
+        ids_h = ids.cpu().numpy()
         for i in range(n):
             a0, a1 = int(step_off[i]), int(step_off[i] + steps[i])
             p0, p1 = int(point_off[i]), int(point_off[i] + points[i])
-            trj = Trajectory(env_id=int(ids[i]), num_units=self._num_units, time_step_length=self._step_length)
+            trj = Trajectory(env_id=int(ids_h[i]), num_units=self._num_units, time_step_length=self._step_length)
             trj.observations = [host["obs"][a0:a1], final_obs[i][None, :]]
             trj.actions = [host["act"][a0:a1, :s.num_actions]]
             trj.rewards = [host["rew"][a0:a1].reshape(-1, 1)]

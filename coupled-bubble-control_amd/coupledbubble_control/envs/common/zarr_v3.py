@@ -14,6 +14,7 @@ arrays.  Arrays are written as one chunk.
 from __future__ import annotations
 
 import json
+import os
 import shutil
 from pathlib import Path
 from typing import Any, Dict, List, Union
@@ -38,9 +39,8 @@ def _jsonable(v: Any) -> Any:
 
 
 def _write_json(path: Path, doc: Dict) -> None:
-    tmp = path.with_suffix(".json.tmp")
-    tmp.write_text(json.dumps(_jsonable(doc), indent=2))
-    tmp.replace(path)
+    with open(path, "w") as f:
+        f.write(json.dumps(_jsonable(doc), separators=(",", ":")))
 
 
 def is_node(path: PathLike) -> bool:
@@ -96,9 +96,13 @@ def write_array(parent: PathLike, name: str, data, dtype: str = "float32") -> Pa
         raise ValueError(f"unsupported dtype {dtype}")
     a = np.ascontiguousarray(np.asarray(data).astype(np.dtype(_DTYPES[dtype]), copy=False))
     p = Path(parent) / name
-    delete(p)
-    p.mkdir(parents=True)
+    if p.exists():
+        delete(p)
     shape = list(a.shape)
+    key = p / "c"
+    for _ in shape:
+        key = key / "0"
+    os.makedirs(key.parent if a.size else p, exist_ok=True)
     doc = {
         "zarr_format": 3,
         "node_type": "array",
@@ -113,12 +117,45 @@ def write_array(parent: PathLike, name: str, data, dtype: str = "float32") -> Pa
     }
     _write_json(p / "zarr.json", doc)
     if a.size:
-        key = p / "c"
-        for _ in shape:
-            key = key / "0"
-        key.parent.mkdir(parents=True, exist_ok=True)
-        key.write_bytes(a.tobytes(order="C"))
+        with open(key, "wb") as f:
+            f.write(a.tobytes(order="C"))
     return p
+
+
+def _array_doc(shape, dtype: str) -> str:
+    fill = "false" if dtype == "bool" else ("0.0" if dtype.startswith("float") else "0")
+    chunk = ",".join(str(max(1, d)) for d in shape)
+    return ('{"zarr_format":3,"node_type":"array","shape":[%s],"data_type":"%s",'
+            '"chunk_grid":{"name":"regular","configuration":{"chunk_shape":[%s]}},'
+            '"chunk_key_encoding":{"name":"default","configuration":{"separator":"/"}},'
+            '"fill_value":%s,"codecs":[{"name":"bytes","configuration":{"endian":"little"}}],'
+            '"attributes":{},"storage_transformers":[]}' % (",".join(str(d) for d in shape), dtype, chunk, fill))
+
+
+def write_new_group(path: PathLike, arrays: Dict[str, np.ndarray], attrs: Dict, dtype: str = "float32") -> None:
+    """A new group at ``path`` (which must not exist) holding ``arrays`` (each cast
+    to ``dtype``, one chunk) and ``attrs``: the same nodes as open_group +
+    write_array + update_attrs, with one directory and file per node part."""
+    base = os.fspath(path)
+    os.mkdir(base)
+    npdt = np.dtype(_DTYPES[dtype])
+    for name, data in arrays.items():
+        a = np.ascontiguousarray(np.asarray(data).astype(npdt, copy=False))
+        node = os.path.join(base, name)
+        os.mkdir(node)
+        with open(os.path.join(node, "zarr.json"), "w") as f:
+            f.write(_array_doc(a.shape, dtype))
+        if a.size:
+            d = os.path.join(node, "c")
+            os.mkdir(d)
+            for _ in range(a.ndim - 1):
+                d = os.path.join(d, "0")
+                os.mkdir(d)
+            with open(os.path.join(d, "0"), "wb") as f:
+                f.write(a.tobytes(order="C"))
+    with open(os.path.join(base, "zarr.json"), "w") as f:
+        f.write(json.dumps({"zarr_format": 3, "node_type": "group", "attributes": _jsonable(attrs)},
+                           separators=(",", ":")))
 
 
 def read_array(path: PathLike) -> np.ndarray:

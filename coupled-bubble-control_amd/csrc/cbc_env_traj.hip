@@ -31,7 +31,8 @@ int32_t report_error(int32_t code, const char* msg);
 
 namespace {
 
-constexpr int kAppendThreads = 64;    // one wave per env
+constexpr int kAppendThreads = 256;   // 4 waves of 8 envs x 8 points
+constexpr int kEnvsPerWave = 8;
 constexpr int kGatherThreads = 256;   // one workgroup per finished env
 
 int32_t fail(int32_t code, const char* fmt, ...)
@@ -58,10 +59,16 @@ struct GatherArgs {
     cbc_traj_packed out;
 };
 
+// A wave covers 8 envs x 8 points: lane = point * 8 + env.  For one point the
+// 8 lanes of consecutive envs read 64 contiguous bytes of the (NDO, NS) float64
+// dense buffers; the 8 lanes of one env write 8 consecutive arena points.
 __global__ __launch_bounds__(kAppendThreads) void traj_append(cbc_traj_spec s, cbc_traj_buffers b)
 {
-    const int e = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & (kEnvsPerWave - 1);          // env within the wave
+    const int pt = lane >> 3;                            // point within a chunk of 8
+    const int e = (blockIdx.x * (kAppendThreads / 64) + (threadIdx.x >> 6)) * kEnvsPerWave + sub;
+    if (e >= s.num_envs) return;
     const size_t NE = (size_t)s.num_envs, N = (size_t)s.num_bubbles, TILE = (size_t)s.tile;
     const int k = b.steps[e];
     const int64_t p0 = b.points[e];
@@ -71,26 +78,25 @@ __global__ __launch_bounds__(kAppendThreads) void traj_append(cbc_traj_spec s, c
     const bool fits = k < s.max_steps && p0 + add <= s.max_points;
     if (fits) {
         const size_t row = (size_t)e * s.max_steps + k;
-        for (int c = lane; c < s.obs_dim; c += kAppendThreads)
-            b.obs[row * s.obs_dim + c] = b.observations[(size_t)e * s.obs_dim + c];
-        for (int c = lane; c < s.num_actions; c += kAppendThreads)
+        for (int c = pt; c < s.obs_dim; c += 8) b.obs[row * s.obs_dim + c] = b.observations[(size_t)e * s.obs_dim + c];
+        for (int c = pt; c < s.num_actions; c += 8)
             b.act[row * s.num_actions + c] = b.actions[(size_t)e * s.num_actions + c];
-        if (lane == 0) b.rew[row] = b.rewards[e];
+        if (pt == 0) b.rew[row] = b.rewards[e];
         const double toff = (double)k * s.step_length;
-        for (int q = lane; q < add; q += kAppendThreads) {
-            const int p = first + q;
-            const size_t dst = (size_t)(p0 + q);
-            const double t = b.dense_time[(size_t)p * NE + e];
-            b.time[(size_t)e * s.max_points + dst] = (float)(k == 0 ? t : t + toff);
-            for (int j = 0; j < s.num_saved; ++j) {
-                const double* src = b.dense_state + (((size_t)p * 4 + s.saved_rows[j]) * NE + e) * TILE;
-                float* d = b.states + (((size_t)e * s.num_saved + j) * s.max_points + dst) * N;
-                for (size_t i = 0; i < N; ++i) d[i] = (float)src[i];
+        float* tdst = b.time + (size_t)e * s.max_points + p0;
+        for (int q = pt; q < add; q += 8) {
+            const double t = b.dense_time[(size_t)(first + q) * NE + e];
+            tdst[q] = (float)(k == 0 ? t : t + toff);
+        }
+        for (int j = 0; j < s.num_saved; ++j) {
+            float* d = b.states + (((size_t)e * s.num_saved + j) * s.max_points + p0) * N;
+            for (int q = pt; q < add; q += 8) {
+                const double* src = b.dense_state + (((size_t)(first + q) * 4 + s.saved_rows[j]) * NE + e) * TILE;
+                for (size_t i = 0; i < N; ++i) d[(size_t)q * N + i] = (float)src[i];
             }
         }
     }
-    __syncthreads();
-    if (lane == 0) {
+    if (pt == 0) {   // the counters were read above by every lane of this env (same wave)
         if (fits) {
             b.steps[e] = k + 1;
             b.points[e] = p0 + add;
@@ -164,8 +170,9 @@ int32_t cbc_traj_append(const cbc_traj_spec* spec, const cbc_traj_buffers* buf, 
     cbc_traj_spec s = *spec;
     cbc_traj_buffers b = *buf;
     void* args[] = {&s, &b};
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&traj_append), dim3(s.num_envs),
-                                   dim3(kAppendThreads), args, 0, (hipStream_t)stream);
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&traj_append),
+                                   dim3((s.num_envs + 4 * kEnvsPerWave - 1) / (4 * kEnvsPerWave)), dim3(kAppendThreads),
+                                   args, 0, (hipStream_t)stream);
     return e == hipSuccess ? CBC_OK : hip_fail(e, "traj_append launch");
 }
 

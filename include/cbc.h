@@ -493,7 +493,7 @@ typedef struct cbc_traj_packed {
     int64_t total_points;
 } cbc_traj_packed;
 
-/* TrajectorBuffer.step (:95-127) for every env; one wave per env.  Asynchronous. */
+/* TrajectorBuffer.step (:95-127) for every env; one wave per 8 envs.  Asynchronous. */
 int32_t cbc_traj_append(const cbc_traj_spec *spec, const cbc_traj_buffers *buf, void *stream);
 
 /* The arenas of envs ids[0..n) into `out` (step_off / point_off: device int64
