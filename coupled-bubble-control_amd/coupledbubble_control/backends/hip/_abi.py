@@ -16,7 +16,7 @@ import torch  # noqa: F401  (load torch's HIP runtime before libcbc)
 _PKG_ROOT = Path(__file__).resolve().parents[3]
 LIB_PATH = Path(os.environ.get("CBC_LIB", _PKG_ROOT / "lib" / "libcbc.so"))
 
-CBC_ABI_VERSION = 1
+CBC_ABI_VERSION = 2
 
 CBC_OK = 0
 CBC_ERR_INVALID = -1
@@ -42,6 +42,10 @@ CBC_MAP_LANE = 1
 CBC_MAP_GROUP = 2
 
 CBC_LIN_DIRECT = 0
+
+CBC_EVENT_BRACKET = 0
+CBC_EVENT_EXACT = 1
+EVENT_MODES = {"bracket": CBC_EVENT_BRACKET, "exact": CBC_EVENT_EXACT}
 
 CBC_SERVE_RESET = 1
 
@@ -69,7 +73,7 @@ class ExecSpecC(ctypes.Structure):
 class SolverSpecC(ctypes.Structure):
     _fields_ = [("atol", _f64), ("rtol", _f64), ("min_step", _f64), ("max_step", _f64), ("etol", _f64),
                 ("lin_atol", _f64), ("lin_rtol", _f64), ("lin_relaxation", _f64),
-                ("lin_max_iter", _i32), ("linsolve", _i32), ("col_threshold", _f64)]
+                ("lin_max_iter", _i32), ("linsolve", _i32), ("col_threshold", _f64), ("event_mode", _i32)]
 
 
 class BuffersC(ctypes.Structure):

@@ -55,7 +55,7 @@ class HipKernel:
         s = _abi.SolverSpecC(solver_spec.atol, solver_spec.rtol, solver_spec.min_step, solver_spec.max_step,
                              solver_spec.etol, solver_spec.lin_atol, solver_spec.lin_rtol,
                              solver_spec.lin_relaxation, solver_spec.lin_max_iter, _abi.CBC_LIN_DIRECT,
-                             solver_spec.col_threshold)
+                             solver_spec.col_threshold, _abi.EVENT_MODES[solver_spec.event_mode])
         handle = ctypes.c_void_p()
         _abi.check(lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(handle)))
         self._handle = handle

@@ -77,6 +77,11 @@ class SolverSpec:
     # -- additions --
     col_threshold: float = 0.1
     linsolve: Literal["direct"] = "direct"
+    # collision events: "bracket" (the reference GPU / Numba rule, |g| < etol
+    # approached by shrinking rejected steps) or "exact" (the root of the gap on
+    # the step's Hermite interpolant, as the reference's CPU path with solve_ivp
+    # events; use with a small etol such as 1e-12)
+    event_mode: Literal["bracket", "exact"] = "bracket"
 
 
 @dataclass(frozen=True)

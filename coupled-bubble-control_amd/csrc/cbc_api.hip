@@ -142,6 +142,8 @@ int32_t cbc_plan_create(const cbc_model_spec* model, const cbc_exec_spec* exec, 
             return fail(CBC_ERR_INVALID, "need 0 < min_step <= max_step");
         if (solver->linsolve != CBC_LIN_DIRECT)
             return fail(CBC_ERR_UNSUPPORTED, "only the direct coupling solve (CBC_LIN_DIRECT) is built");
+        if (solver->event_mode != CBC_EVENT_BRACKET && solver->event_mode != CBC_EVENT_EXACT)
+            return fail(CBC_ERR_INVALID, "unknown event mode %d", solver->event_mode);
 
         // AUTO: whole system in one lane while it fits in VGPRs (N <= 2), else one
         // lane per bubble with the coupling solve staged in LDS
@@ -165,6 +167,9 @@ int32_t cbc_plan_create(const cbc_model_spec* model, const cbc_exec_spec* exec, 
         } else {
             return fail(CBC_ERR_INVALID, "unknown mapping %d", mapping);
         }
+        if (solver->event_mode == CBC_EVENT_EXACT && mapping == CBC_MAP_GROUP && exec->ups > 4)
+            return fail(CBC_ERR_UNSUPPORTED, "the exact event mode is built for up to 4 bubbles per system, got %d",
+                        exec->ups);
 
         cbc_plan* p = new (std::nothrow) cbc_plan();
         if (!p) return fail(CBC_ERR_NOMEM, "cbc_plan_create: out of host memory");
@@ -250,6 +255,7 @@ static int32_t fill_args(const cbc_plan* p, const cbc_buffers* b, cbc::LaunchArg
     a.max_step = p->solver.max_step;
     a.etol = p->solver.etol;
     a.col_factor = 1.0 + p->solver.col_threshold;
+    a.event_exact = p->solver.event_mode == CBC_EVENT_EXACT;
     a.t = b->actual_time;
     a.t_end = b->time_end;
     a.t_begin = b->time_begin;

@@ -249,4 +249,50 @@ __device__ __forceinline__ double step_factor(bool ok, double ratio)
     return ok ? fmin(5.0, fmax(0.2, f)) : fmin(0.5, fmax(0.1, f));
 }
 
+// ---------------------------------------------------------------------------
+// Exact collision-event localisation (CBC_EVENT_EXACT).  Along one step of
+// length h the radii and positions follow the cubic Hermite interpolant of their
+// end values with slopes h U and h V (R' = U and x' = V in the dimensionless
+// model), so the gap is known on the whole step without another RHS
+// evaluation; its root theta in (0, 1) is found by the Illinois method
+// (modified regula falsi, Dowell & Jarratt 1971) from the bracket
+// (0, g_old), (1, g_new).  The iteration count is fixed at most and a
+// converged iterate is frozen, so every mapping and the oracle reach the same
+// theta from the same states.
+
+constexpr int kIllinoisIters = 60;
+constexpr double kEventThetaMax = 0.99;   // a retried step always shrinks
+
+__device__ __forceinline__ double hermite3(double p0, double p1, double d0, double d1, double th)
+{
+    const double t2 = th * th, t3 = t2 * th;
+    const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0;
+    const double h10 = t3 - 2.0 * t2 + th;
+    const double h01 = 3.0 * t2 - 2.0 * t3;
+    const double h11 = t3 - t2;
+    return h00 * p0 + h10 * d0 + h01 * p1 + h11 * d1;
+}
+
+struct Illinois {
+    double a, fa, b, fb;
+    bool done;
+};
+
+__device__ __forceinline__ double illinois_point(const Illinois& s) { return s.b - s.fb * (s.b - s.a) / (s.fb - s.fa); }
+
+__device__ __forceinline__ void illinois_update(Illinois& s, double c, double fc)
+{
+    if (s.done) return;
+    const bool conv = fc == 0.0 || fabs(c - s.b) <= 1e-15;
+    if (fc * s.fb < 0.0) {
+        s.a = s.b;
+        s.fa = s.fb;
+    } else {
+        s.fa *= 0.5;
+    }
+    s.b = c;
+    s.fb = fc;
+    s.done = conv;
+}
+
 }  // namespace cbc

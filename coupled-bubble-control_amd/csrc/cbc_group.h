@@ -287,4 +287,26 @@ __device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, do
     }
 }
 
+// Root theta of the gap along the step y0 -> y1 of length h (exact event mode),
+// identical on every lane of the group.  Every lane of the wave runs all
+// iterations (the gap evaluation has barriers); a group that converged keeps
+// its iterate, so the result is the early-exit one of the lane mapping.
+template <int N>
+__device__ double group_event_root(GroupLds<N>& L, int i, bool vu, double rcol, const double (&y0)[4],
+                                   const double (&y1)[4], double h, double colf, double g0, double g1)
+{
+    Illinois st{0.0, g0, 1.0, g1, false};
+    for (int it = 0; it < kIllinoisIters; ++it) {
+        const double c = illinois_point(st);
+        double yc[4];
+        yc[0] = hermite3(y0[0], y1[0], h * y0[2], h * y1[2], c);
+        yc[1] = hermite3(y0[1], y1[1], h * y0[3], h * y1[3], c);
+        yc[2] = 0.0;
+        yc[3] = 0.0;
+        const double g = group_event<N>(L, i, vu, rcol, yc, colf);
+        illinois_update(st, c, g);
+    }
+    return st.b;
+}
+
 }  // namespace cbc

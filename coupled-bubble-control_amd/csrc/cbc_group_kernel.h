@@ -251,17 +251,33 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     }
     const double ratio = group_min<G>(ratio_u);
     const bool bad = group_any<G>(bad_u);
+    // exact event mode (N <= 4 only: the Gauss--Jordan kernels for N = 8, 16 are
+    // built without it): the gap's root on the step, for groups that cross
+    constexpr bool kExact = GroupShape<N>::redundant;
+    double theta = 1.0;
+    if constexpr (kExact) {
+        if (a.event_exact) {
+            const bool need = busy && R.g_old * g_new < 0.0 && !(fabs(g_new) < a.etol);
+            if (__any(need))
+                theta = group_event_root<N>(L, i, vu, R.P.rcol, y, ynew, h, a.col_factor, R.g_old, g_new);
+        }
+    }
     if (!busy) return false;
 
     const int s = R.s;
-    const bool crossing = R.g_old * g_new < 0.0;
+    bool crossing = R.g_old * g_new < 0.0;
+    if constexpr (kExact) crossing = crossing && !(a.event_exact && fabs(g_new) < a.etol);
     if (crossing) {
         if (h <= a.min_step) {
             R.flag = kStepSizeLimit;
             return true;
         }
-        const double aold = fabs(R.g_old);
-        h = h * ((aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99);
+        if (kExact && a.event_exact) {
+            h = h * fmin(theta, kEventThetaMax);
+        } else {
+            const double aold = fabs(R.g_old);
+            h = h * ((aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99);
+        }
         ++R.rejected;
     } else {
         if (bad) {

@@ -14,6 +14,7 @@ struct LaunchArgs {
     int64_t kernel_steps;      // attempts per system in this launch
     double atol, rtol, min_step, max_step, etol;
     double col_factor;         // 1 + collision threshold
+    int event_exact;           // CBC_EVENT_EXACT
 
     double* t;
     const double* t_end;

@@ -169,6 +169,31 @@ __device__ __forceinline__ double lane_event(const LaneSys<N, NK>& P, const doub
     return g;
 }
 
+// Root theta of the gap along the step y0 -> y1 of length h (exact event mode).
+template <int N, int NK>
+__device__ double lane_event_root(const LaneSys<N, NK>& P, const double (&y0)[4 * N], const double (&y1)[4 * N],
+                                  double h, double colf, double g0, double g1)
+{
+    Illinois st{0.0, g0, 1.0, g1, false};
+    for (int it = 0; it < kIllinoisIters && !st.done; ++it) {
+        const double c = illinois_point(st);
+        double g = 1e30;
+        double r[N], x[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            r[i] = hermite3(y0[i], y1[i], h * y0[2 * N + i], h * y1[2 * N + i], c) * P.rcol[i];
+            x[i] = hermite3(y0[N + i], y1[N + i], h * y0[3 * N + i], h * y1[3 * N + i], c);
+        }
+#pragma unroll
+        for (int i = 0; i + 1 < N; ++i) {
+            const double gap = fabs(x[i] - x[i + 1]) - (r[i] + r[i + 1]) * colf;
+            if (gap < g) g = gap;
+        }
+        illinois_update(st, c, g);
+    }
+    return st.b;
+}
+
 // Per-lane integration state of one system.
 template <int N, int NK>
 struct LaneRun {
@@ -411,15 +436,21 @@ __device__ __forceinline__ bool lane_attempt(const LaunchArgs& a, LaneRun<N, NK>
     for (int c = 0; c < 4 * N; ++c) ynew[c] = y[c] + h * ysum[c];
     const double g_new = lane_event<N, NK>(P, ynew, a.col_factor);
 
-    // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581)
-    const bool crossing = R.g_old * g_new < 0.0;
+    // event crossing (rkck45_coupled_numba_stepper.py:500-515, 575-581); in the
+    // exact mode a crossing step that lands within etol of the root is accepted
+    const bool crossing = R.g_old * g_new < 0.0 && !(a.event_exact && fabs(g_new) < a.etol);
     if (crossing) {
         if (h <= a.min_step) {  // cannot shrink further: no progress possible
             R.flag = kStepSizeLimit;
             return true;
         }
-        const double aold = fabs(R.g_old);
-        const double ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
+        double ratio;
+        if (a.event_exact) {
+            ratio = fmin(lane_event_root<N, NK>(P, y, ynew, h, a.col_factor, R.g_old, g_new), kEventThetaMax);
+        } else {
+            const double aold = fabs(R.g_old);
+            ratio = (aold * rcp_fast(aold + fabs(g_new) + 1.0e-15)) * 0.99;
+        }
         h = h * ratio;
         ++R.rejected;
     } else {

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CBC_ABI_VERSION 1
+#define CBC_ABI_VERSION 2
 
 /* return codes */
 #define CBC_OK 0
@@ -76,14 +76,30 @@ typedef struct cbc_exec_spec {
     int32_t block_size;    /* threads per block, 0 = plan default */
 } cbc_exec_spec;
 
+/* Collision-event localisation.
+ * BRACKET: the reference GPU / Numba rule (rkck45_coupled_numba_stepper.py:500-515,
+ *   .cu:608-641): a step across a sign change of the gap is rejected and retried
+ *   with h * 0.99 |g_old| / (|g_old| + |g_new|) until |g| < etol.
+ * EXACT: the root of the gap as the reference's CPU path finds it with
+ *   solve_ivp events (CKM1D_reference.py:702-714, 848-856): the rejected step is
+ *   retried with h * theta, theta the root of the gap along the step's cubic
+ *   Hermite interpolant of R and x (slopes U and V, the state's own derivatives),
+ *   and a step that crosses with |g_new| < etol is accepted; with etol ~1e-12
+ *   the run ends at the root to that tolerance.  Built for the lane mapping and
+ *   for the group mapping up to 4 bubbles; cbc_plan_create returns
+ *   CBC_ERR_UNSUPPORTED for EXACT with more bubbles. */
+#define CBC_EVENT_BRACKET 0
+#define CBC_EVENT_EXACT 1
+
 /* SolverSpec, models/specs.py:55-71, plus the collision threshold of
- * ckm1d_def.py:534 and the solve selector */
+ * ckm1d_def.py:534, the solve selector and the event mode */
 typedef struct cbc_solver_spec {
     double atol, rtol, min_step, max_step, etol;
     double lin_atol, lin_rtol, lin_relaxation;
     int32_t lin_max_iter;
     int32_t linsolve;       /* CBC_LIN_* */
     double col_threshold;
+    int32_t event_mode;     /* CBC_EVENT_* */
 } cbc_solver_spec;
 
 /* The 14 kernel buffers of rkck45_coupled_cuda_sover.cu:206-226 plus the two

@@ -67,6 +67,7 @@ typedef struct {
 
 typedef struct {
     double atol, rtol, min_step, max_step, etol;
+    int event_exact;      /* exact collision-event localisation (see or_event_root) */
 } or_tol;
 
 /* ---------------------------------------------------------------------- */
@@ -299,6 +300,44 @@ double or_event(const or_system *S, const double *y)
     return g;
 }
 
+/*
+ * Exact event mode: the root of the collision gap along one step y0 -> y1 of
+ * length h.  R and x follow cubic Hermite interpolants of their end values
+ * with slopes h*U and h*V (R' = U, x' = V in the dimensionless model); the gap
+ * of the interpolated state is solved for by the Illinois method (modified
+ * regula falsi, Dowell & Jarratt 1971) from the bracket (0, g0), (1, g1): at
+ * most 60 iterations, stopping when the gap is 0 or two iterates differ by at
+ * most 1e-15.  The reference's CPU path locates the same root with solve_ivp's
+ * event search on its dense output (CKM1D_reference.py:702-714, 848-856).
+ */
+static double or_hermite3(double p0, double p1, double d0, double d1, double th)
+{
+    double t2 = th * th, t3 = t2 * th;
+    return (2.0 * t3 - 3.0 * t2 + 1.0) * p0 + (t3 - 2.0 * t2 + th) * d0 + (3.0 * t2 - 2.0 * t3) * p1
+           + (t3 - t2) * d1;
+}
+
+double or_event_root(const or_system *S, const double *y0, const double *y1, double h, double g0, double g1)
+{
+    const int n = S->n;
+    double yc[2 * OR_MAXN];
+    double a = 0.0, fa = g0, b = 1.0, fb = g1;
+    for (int it = 0; it < 60; it++) {
+        double c = b - fb * (b - a) / (fb - fa);
+        for (int i = 0; i < n; i++) {
+            yc[i] = or_hermite3(y0[i], y1[i], h * y0[2 * n + i], h * y1[2 * n + i], c);
+            yc[n + i] = or_hermite3(y0[n + i], y1[n + i], h * y0[3 * n + i], h * y1[3 * n + i], c);
+        }
+        double fc = or_event(S, yc);
+        int conv = (fc == 0.0) || fabs(c - b) <= 1e-15;
+        if (fc * fb < 0.0) { a = b; fa = fb; } else { fa *= 0.5; }
+        b = c; fb = fc;
+        if (conv) break;
+    }
+    return b;
+}
+
+
 /* Cash--Karp tableau: rkck45_coupled_cuda_sover.cu:137-147 */
 static const double CK_A[6] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 3.0 / 5.0, 1.0, 7.0 / 8.0};
 static const double CK_B[6][5] = {
@@ -454,10 +493,13 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
         }
         double g_new = or_event(S, ynew);
 
-        /* event handling: stepper.py:500-515 */
+        /* event handling: stepper.py:500-515; the exact mode accepts a crossing
+           step within etol of the root and otherwise aims the retry at the root */
         int crossing = (g_old * g_new < 0.0);
+        if (crossing && T->event_exact && fabs(g_new) < T->etol) crossing = 0;
         double aold = fabs(g_old);
         double ev_ratio = (aold / (aold + fabs(g_new) + 1e-15)) * 0.99;
+        if (crossing && T->event_exact) ev_ratio = fmin(or_event_root(S, y, ynew, h, g_old, g_new), 0.99);
 
         int nan_state = 0;
         for (int c = 0; c < d; c++) if (!isfinite(ynew[c])) nan_state = 1;
@@ -566,6 +608,7 @@ typedef struct {
     double atol, rtol, min_step, max_step, etol;
     int linsolve; double lin_atol, lin_rtol, lin_relax; int lin_max_iter;
     double col_threshold;
+    int event_exact;
 } or_config;
 
 static void or_load_system(const or_config *C, int s, const double *unit_params,
@@ -607,7 +650,7 @@ int or_solve_reference_layout(const or_config *C, int kernel_steps, int64_t max_
 {
     const int NS = C->ns, TILE = C->tile, n = C->ups, NDO = C->ndo;
     const int64_t max_launches = (max_steps + kernel_steps - 1) / kernel_steps;
-    or_tol T = {C->atol, C->rtol, C->min_step, C->max_step, C->etol};
+    or_tol T = {C->atol, C->rtol, C->min_step, C->max_step, C->etol, C->event_exact};
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+:bad)
     for (int s = 0; s < NS; s++) {
@@ -674,7 +717,7 @@ int or_solve_periodic(const or_config *C, int64_t max_attempts, double period, i
                       double *max_r, double *mean_x, int s_begin, int s_end)
 {
     const int NS = C->ns, TILE = C->tile, n = C->ups;
-    or_tol T = {C->atol, C->rtol, C->min_step, C->max_step, C->etol};
+    or_tol T = {C->atol, C->rtol, C->min_step, C->max_step, C->etol, C->event_exact};
     int bad = 0;
     if (s_end > NS) s_end = NS;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+:bad)

@@ -29,11 +29,12 @@ class _Config(ctypes.Structure):
     _fields_ = [("ns", _i32), ("ups", _i32), ("tile", _i32), ("nk", _i32), ("field", _i32), ("ndo", _i32),
                 ("atol", _f64), ("rtol", _f64), ("min_step", _f64), ("max_step", _f64), ("etol", _f64),
                 ("linsolve", _i32), ("lin_atol", _f64), ("lin_rtol", _f64), ("lin_relax", _f64),
-                ("lin_max_iter", _i32), ("col_threshold", _f64)]
+                ("lin_max_iter", _i32), ("col_threshold", _f64), ("event_exact", _i32)]
 
 
 class _Tol(ctypes.Structure):
-    _fields_ = [("atol", _f64), ("rtol", _f64), ("min_step", _f64), ("max_step", _f64), ("etol", _f64)]
+    _fields_ = [("atol", _f64), ("rtol", _f64), ("min_step", _f64), ("max_step", _f64), ("etol", _f64),
+                ("event_exact", _i32)]
 
 
 _lib = None
@@ -91,6 +92,7 @@ class OracleSpec:
     lin_relax: float = 1.0
     lin_max_iter: int = 100
     col_threshold: float = 0.1
+    event_mode: str = "bracket"     # "bracket" (reference GPU / Numba rule) or "exact" (or_event_root)
 
     @property
     def tile(self) -> int:
@@ -99,7 +101,7 @@ class OracleSpec:
     def c(self) -> _Config:
         return _Config(self.ns, self.ups, self.tile, self.nk, FIELD[self.field], self.ndo, self.atol, self.rtol,
                        self.min_step, self.max_step, self.etol, LIN[self.linsolve], self.lin_atol, self.lin_rtol,
-                       self.lin_relax, self.lin_max_iter, self.col_threshold)
+                       self.lin_relax, self.lin_max_iter, self.col_threshold, int(self.event_mode == "exact"))
 
 
 def spec_from_model(model, linsolve: str = "direct", lin_relax: float = 1.0) -> OracleSpec:
@@ -107,7 +109,8 @@ def spec_from_model(model, linsolve: str = "direct", lin_relax: float = 1.0) -> 
     return OracleSpec(ns=es.ns, ups=es.ups, nk=ms.nk, field=ms.ac, ndo=es.ndo, atol=ss.atol, rtol=ss.rtol,
                       min_step=ss.min_step, max_step=ss.max_step, etol=ss.etol, linsolve=linsolve,
                       lin_atol=ss.lin_atol, lin_rtol=ss.lin_rtol, lin_relax=lin_relax,
-                      lin_max_iter=ss.lin_max_iter, col_threshold=ss.col_threshold)
+                      lin_max_iter=ss.lin_max_iter, col_threshold=ss.col_threshold,
+                      event_mode=getattr(ss, "event_mode", "bracket"))
 
 
 def _c64(a):

@@ -60,3 +60,23 @@ def test_plan_create_rejects_bad_specs_without_gpu_work():
     rc = lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h))
     assert rc == _abi.CBC_ERR_INVALID
     assert lib.cbc_launch(None, None, 1, None) == _abi.CBC_ERR_INVALID
+
+
+def test_plan_create_rejects_bad_event_modes_without_gpu_work():
+    """An unknown event mode is invalid; the exact mode is built for up to 4
+    bubbles per system in the group mapping."""
+    from coupledbubble_control.backends.hip import _abi
+
+    lib = _abi.lib()
+    m = _abi.ModelSpecC(4, 13, 5, 3, 2, 1, 4, 8, _abi.CBC_FIELD_SW_N)
+    e = _abi.ExecSpecC(16, 1, 2, 0, 2, _abi.CBC_MAP_AUTO, 0)
+    s = _abi.SolverSpecC(1e-9, 1e-9, 1e-16, 0.1, 1e-12, 1e-9, 1e-9, 2 / 3, 100, _abi.CBC_LIN_DIRECT, 0.1, 7)
+    h = ctypes.c_void_p()
+    assert lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h)) == \
+        _abi.CBC_ERR_INVALID
+    assert b"event mode" in lib.cbc_last_error()
+    e = _abi.ExecSpecC(16, 1, 8, 0, 8, _abi.CBC_MAP_AUTO, 0)
+    s.event_mode = _abi.CBC_EVENT_EXACT
+    assert lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h)) == \
+        _abi.CBC_ERR_UNSUPPORTED
+    assert b"up to 4 bubbles" in lib.cbc_last_error()

@@ -24,7 +24,12 @@ def _device_rows(m):
 
 def _assert_equal_rows(a, b, what):
     for k in a:
-        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}: {k}")
+        bad = np.flatnonzero((a[k] != b[k]).reshape(a[k].shape[0], -1).any(axis=1)) if a[k].ndim == 1 else \
+            np.flatnonzero((a[k] != b[k]).any(axis=(0, 2)))
+        if bad.size:
+            rows = [f"{s}: flags {a['flags'][s]}/{b['flags'][s]} t {a['t'][s]!r}/{b['t'][s]!r} "
+                    f"h {a['h'][s]!r}/{b['h'][s]!r} steps {a['steps'][s]}/{b['steps'][s]}" for s in bad[:6]]
+            raise AssertionError(f"{what}: {k} differs for {bad.size} systems, e.g.\n  " + "\n  ".join(rows))
 
 
 def _oracle_sample(w, idx, rows, n):
