@@ -94,6 +94,9 @@ def parse():
                    help="env: record trajectories (dense output, device arena, zarr files in a temporary "
                         "directory); 2 writes the files on a background thread")
     p.add_argument("--env-episode-length", type=int, default=128, help="env: steps per episode")
+    p.add_argument("--env-policy", type=int, default=0,
+                   help="env: choose actions with an actor-critic MLP forward (256x256 tanh, Beta mean) and "
+                        "evaluate the critic on the next observations each step, as a PPO rollout does")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample length")
     return p.parse_args()
@@ -541,9 +544,46 @@ def bench_env(args):
             end_episode(*a, **kw)
             write_s.append(time.perf_counter() - c0)
         tb.end_episode = timed_end
+    policy = None
+    if args.env_policy:
+        A, D = env.action_space.space_data.shape[0], env.observation_space.space_data.shape[0]
+
+        def mlp(out):
+            return torch.nn.Sequential(torch.nn.Linear(D, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
+                                       torch.nn.Tanh(), torch.nn.Linear(256, out)).cuda()
+        actor, critic = mlp(2 * A), mlp(1)
+        obs_now = [env._observations]
+
+        def policy():
+            with torch.no_grad():
+                ab = torch.nn.functional.softplus(actor(obs_now[0])) + 1.0
+                return ab[:, :A] / (ab[:, :A] + ab[:, A:])
+
+        def value(o):
+            with torch.no_grad():
+                return critic(o)
+    ev["policy"] = []
+
+    def step_once():
+        if not args.env_policy:
+            return env.step(None)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        act = policy()
+        e1.record(stream)
+        r = env.step(act)
+        e2, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e2.record(stream)
+        value(r[0])
+        e3.record(stream)
+        ev["policy"].append((e0, e1))
+        ev["policy"].append((e2, e3))
+        obs_now[0] = r[0]
+        return r
+
     log(f"env: {args.warmup} warmup steps")
     for _ in range(args.warmup):
-        env.step(None)
+        step_once()
     torch.cuda.synchronize()
     for v in ev.values():
         v.clear()
@@ -555,7 +595,7 @@ def bench_env(args):
     t0 = time.perf_counter()
     for k in range(steps):
         s0 = time.perf_counter()
-        info = env.step(None)[4]
+        info = step_once()[4]
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - s0)
         dones += int(info["dones"].numel()) if info else 0
@@ -585,6 +625,7 @@ def bench_env(args):
                                                  - (np.sum(rst) / steps if rst else 0.0))},
         "episodes_finished": dones,
         "episode_length": args.env_episode_length,
+        **({"policy_forward_ms_per_step": float(np.sum(ms("policy")) / steps)} if args.env_policy else {}),
         "roofline": roofline(flops, float(np.sum(phys)) / 1e3, window="solver launches of the timed steps",
                              attempts={str(n): att}),
     }
