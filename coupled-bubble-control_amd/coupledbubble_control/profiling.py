@@ -8,7 +8,8 @@
                   and bubble count: one plain launch integrating one period of
                   a serving-style ensemble, kernel time (HIP events, best of 3)
                   over the largest attempt count of any system.  Prints one
-                  JSON object per case.
+                  JSON object per case; ``--systems`` reruns every case at the
+                  given ensemble sizes (occupancy sweep).
 * ``param``    -- ``--steps`` launches of the parameter-study kernel
                   (65 536 systems of N=2, 10 periods each), the dominant
                   kernel of that workload, for counter collection.
@@ -116,6 +117,8 @@ def main(argv: Sequence[str] = None) -> None:
     p.add_argument("--mib", type=int, default=512)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--cases", default="", help="latency: comma list like 2lane,4group (default: all)")
+    p.add_argument("--systems", default="",
+                   help="latency: comma list of system counts run for every case (default: each case's own)")
     a = p.parse_args(argv)
     if a.what == "copy":
         print(f"copy kernel: {copy_kernel(a.mib)} bytes read and written per launch", flush=True)
@@ -126,6 +129,8 @@ def main(argv: Sequence[str] = None) -> None:
         if a.cases:
             want = set(a.cases.split(","))
             cases = [c for c in LATENCY_CASES if f"{c[0]}{c[1]}" in want]
+        if a.systems:
+            cases = [(n, var, int(ns)) for n, var, _ in cases for ns in a.systems.split(",")]
         for r in attempt_latency(cases):
             print(json.dumps(r), flush=True)
 

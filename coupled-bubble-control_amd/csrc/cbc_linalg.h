@@ -9,9 +9,13 @@ namespace cbc {
 // In-register LU with partial pivoting of the augmented n x (n+1) system.
 // The pivot search is a sequence of conditional row exchanges so every index
 // stays a compile-time constant (no scratch).  The first row holding the
-// largest |a_ik| ends up as pivot, as in the CPU restatement.  The coupling
-// matrix is I + C with |C| small, so exchanges are rare: the exchange code
-// runs only when some lane of the wave needs one (wave-uniform branch).
+// largest |a_ik| ends up as pivot, as in the CPU restatement.  The exchange
+// code of a step runs only when some lane of the wave needs an exchange there
+// (wave-uniform branch).  Exchanges are not rare in this model: in a wave of
+// 16 systems of 4 bubbles some system needs one in most evaluations, so an
+// exchange path that is cheap only when seldom taken (a serial solve of the
+// staged rows) made the group kernel 3.6x slower, while the cascade costs ~20 %
+// of an N = 4 attempt.
 template <int n>
 __device__ __forceinline__ void lu_solve(double (&a)[n][n + 1], double (&x)[n])
 {
