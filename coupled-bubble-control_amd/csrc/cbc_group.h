@@ -18,6 +18,8 @@
 // step controller runs redundantly and stays consistent without a broadcast.
 #pragma once
 
+#include <type_traits>
+
 #include "cbc_device.h"
 #include "cbc_kernels.h"
 #include "cbc_linalg.h"
@@ -47,12 +49,48 @@ struct GroupLds {
     double rows[R ? n : 1][n + 1];  // the augmented system (redundant solve)
 };
 
-// Butterfly reductions inside a group of G lanes (every lane gets the result).
+// Reductions inside a group of G <= 16 lanes (every lane gets the result),
+// by DPP row permutations instead of LDS-routed shuffles: partner lane i ^ 1
+// and i ^ 2 (quad_perm), then 7 - i within 8 lanes (row_half_mirror) and
+// 15 - i within 16 (row_mirror).  After the quad steps every lane of a quad
+// holds the quad's result, so each mirror step merges two whole sub-groups.
+// min, OR and the (value, lowest id) maximum do not depend on the order in
+// which elements meet, so the result is the butterfly's, bit for bit.  The
+// reductions run in wave-uniform code (all lanes active); a lane whose source
+// were inactive would keep its own value.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+
+template <int Ctl>
+__device__ __forceinline__ int dpp_int(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, Ctl, 0xF, 0xF, false);
+}
+
+template <int Ctl>
+__device__ __forceinline__ double dpp_double(double v)
+{
+    const int lo = dpp_int<Ctl>(__double2loint(v));
+    const int hi = dpp_int<Ctl>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+template <int G, typename Step>
+__device__ __forceinline__ void group_steps(Step step)
+{
+    static_assert(G >= 1 && G <= 16 && (G & (G - 1)) == 0, "group of 1..16 lanes, a power of two");
+    if constexpr (G >= 2) step(std::integral_constant<int, kDppXor1>{});
+    if constexpr (G >= 4) step(std::integral_constant<int, kDppXor2>{});
+    if constexpr (G >= 8) step(std::integral_constant<int, kDppHalfMirror>{});
+    if constexpr (G >= 16) step(std::integral_constant<int, kDppMirror>{});
+}
+
 template <int G>
 __device__ __forceinline__ double group_min(double v)
 {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v = fmin(v, __shfl_xor(v, m, G));
+    group_steps<G>([&](auto c) { v = fmin(v, dpp_double<decltype(c)::value>(v)); });
     return v;
 }
 
@@ -60,8 +98,7 @@ template <int G>
 __device__ __forceinline__ bool group_any(bool v)
 {
     int x = v ? 1 : 0;
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) x |= __shfl_xor(x, m, G);
+    group_steps<G>([&](auto c) { x |= dpp_int<decltype(c)::value>(x); });
     return x != 0;
 }
 
@@ -69,14 +106,14 @@ __device__ __forceinline__ bool group_any(bool v)
 template <int G>
 __device__ __forceinline__ void group_argmax(double& v, int& id)
 {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) {
-        const double ov = __shfl_xor(v, m, G);
-        const int oid = __shfl_xor(id, m, G);
+    group_steps<G>([&](auto c) {
+        constexpr int ctl = decltype(c)::value;
+        const double ov = dpp_double<ctl>(v);
+        const int oid = dpp_int<ctl>(id);
         const bool take = (ov > v) || (ov == v && oid < id);
         v = take ? ov : v;
         id = take ? oid : id;
-    }
+    });
 }
 
 // Per-lane constants of one bubble and its system.
