@@ -8,9 +8,18 @@ in the reference buffer layout) and integrates all of them together:
 * each bin runs one persistent work-queue launch (cbc_launch_queued): lanes or
   lane groups take that bin's systems longest-horizon-first and refill as
   systems stop, so uneven horizons and step counts do not idle lanes;
-* the bins' launches are issued on separate streams without host
+* the bins' launches are issued on side streams without host
   synchronisation between them, so when one bin's queue drains its waves free
-  SIMDs for the others (the tails overlap);
+  SIMDs for the others (the tails overlap).  A bin's time is bounded below by
+  its slowest system's chain of attempts, which grows with N, so the largest
+  bubble count is issued first, alone on the first side stream, and the other
+  bins go round-robin, largest first, onto at most two more (queued launches
+  on one stream run one after the other).  At most three side streams: with
+  the current stream they fit the four hardware queues of the device, and a
+  launch whose stream shares a hardware queue with another bin's waits for
+  that launch to end.  32 768-system mixed set: 8.0-8.2 s this way; 11.7-12.0 s
+  with one stream per bin and the N = 8 bin on the fourth; the N = 8 bin alone
+  takes 7.8 s;
 * the launch sequence can be captured once into a HIP graph and replayed;
 * ``autotune`` picks, per bin, the mapping (lane vs group), the refill
   threshold and the lane mapping's workgroup size from short timed trials.
@@ -28,6 +37,9 @@ import torch
 
 from .backends import KernelVariant
 from .backends.hip import rkck45_coupled_hip_solver as hs
+
+
+MAX_SIDE_STREAMS = 3   # hardware queues per device (4) minus the current stream's
 
 
 @dataclass
@@ -69,8 +81,9 @@ class MixedEnsemble:
                 m.runtime.variant = want
                 m._hip_opts.block_size = block
                 m.runtime.solver = m._make_solver(m.runtime.backend, want)
-        if len(self._streams) != len(self.models):
-            self._streams = [torch.cuda.Stream() for _ in self.models]
+        k = min(len(self.models), MAX_SIDE_STREAMS)
+        if len(self._streams) != k:
+            self._streams = [torch.cuda.Stream() for _ in range(k)]
 
     def orders(self) -> List[torch.Tensor]:
         """Longest-remaining-horizon-first queues of every bin (kept alive for replays)."""
@@ -84,9 +97,13 @@ class MixedEnsemble:
         self._prepare()
         orders = orders if orders is not None else self.orders()
         main = torch.cuda.current_stream()
-        for m, st, order in zip(self.models, self._streams, orders):
-            t = self.tuning[m._execution_spec.ups]
+        for st in self._streams:
             st.wait_stream(main)
+        k = len(self._streams)
+        issue = sorted(zip(self.models, orders), key=lambda e: -e[0]._execution_spec.ups)
+        for j, (m, order) in enumerate(issue):
+            st = self._streams[0 if j == 0 or k == 1 else 1 + (j - 1) % (k - 1)]
+            t = self.tuning[m._execution_spec.ups]
             with torch.cuda.stream(st):
                 hs.launch_queued(m.runtime.kernel, m.runtime.device_buffers, order, waves=t.waves,
                                  refill=t.refill, kernel_steps=kernel_steps, stream=st)
@@ -99,8 +116,9 @@ class MixedEnsemble:
         for _ in range(max_launches):
             self.launch(kernel_steps)
             rounds += 1
-            torch.cuda.current_stream().synchronize()
-            left = sum(hs.active_count(m.runtime.kernel, st) for m, st in zip(self.models, self._streams))
+            main = torch.cuda.current_stream()
+            main.synchronize()   # joined every side stream in launch()
+            left = sum(hs.active_count(m.runtime.kernel, main) for m in self.models)
             if left == 0:
                 break
         return rounds
