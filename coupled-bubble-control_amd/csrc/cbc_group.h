@@ -7,7 +7,7 @@
 // (translational) of the 2N x (2N+1) augmented system in VGPRs and the group
 // runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
-// butterfly argmax across the group; its owner stages it in LDS (double
+// DPP argmax across the group; its owner stages it in LDS (double
 // buffered, one barrier per column) and every lane eliminates that column from
 // its rows.  For N <= 4 (n <= 8) the group instead stages its n rows in LDS
 // once and every lane solves the whole system in registers with the lane
@@ -55,7 +55,8 @@ struct GroupLds {
 // 15 - i within 16 (row_mirror).  After the quad steps every lane of a quad
 // holds the quad's result, so each mirror step merges two whole sub-groups.
 // min, OR and the (value, lowest id) maximum do not depend on the order in
-// which elements meet, so the result is the butterfly's, bit for bit.  The
+// which elements meet, so the result is a butterfly's (xor partners through
+// LDS-routed shuffles, as these reductions were first written) bit for bit.  The
 // reductions run in wave-uniform code (all lanes active); a lane whose source
 // were inactive would keep its own value.
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
