@@ -42,6 +42,16 @@ from .backends.hip import rkck45_coupled_hip_solver as hs
 MAX_SIDE_STREAMS = 3   # hardware queues per device (4) minus the current stream's
 
 
+def stream_slots(bubble_counts: Sequence[int], streams: int) -> List[tuple]:
+    """Issue order and side stream of each bin: (bin index, stream index) pairs,
+    largest bubble count first; the first bin alone on stream 0, the others
+    round-robin over the remaining streams (all on stream 0 when there is one)."""
+    if streams < 1:
+        raise ValueError("need at least one stream")
+    order = sorted(range(len(bubble_counts)), key=lambda i: -bubble_counts[i])
+    return [(i, 0 if j == 0 or streams == 1 else 1 + (j - 1) % (streams - 1)) for j, i in enumerate(order)]
+
+
 @dataclass
 class BinTuning:
     """Launch choice for one bin."""
@@ -99,10 +109,9 @@ class MixedEnsemble:
         main = torch.cuda.current_stream()
         for st in self._streams:
             st.wait_stream(main)
-        k = len(self._streams)
-        issue = sorted(zip(self.models, orders), key=lambda e: -e[0]._execution_spec.ups)
-        for j, (m, order) in enumerate(issue):
-            st = self._streams[0 if j == 0 or k == 1 else 1 + (j - 1) % (k - 1)]
+        ups = [m._execution_spec.ups for m in self.models]
+        for i, slot in stream_slots(ups, len(self._streams)):
+            m, order, st = self.models[i], orders[i], self._streams[slot]
             t = self.tuning[m._execution_spec.ups]
             with torch.cuda.stream(st):
                 hs.launch_queued(m.runtime.kernel, m.runtime.device_buffers, order, waves=t.waves,
