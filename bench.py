@@ -648,7 +648,7 @@ def bench_env(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the solver runs on one GPU per process; --gpus must be 1 (sharding across GPUs is not built)", "n_gpus": args.gpus}))
         sys.exit(2)
     log("importing torch")
     import torch
