@@ -13,12 +13,14 @@
 // once and every lane solves the whole system in registers with the lane
 // mapping's LU (identical results on every lane, one barrier instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
-// coupling constants mx are staged in LDS per system.  All lanes of a group
+// coupling constants mx are staged in LDS per system; pairs (N = 2) pass the
+// bubble data and the rows by DPP broadcasts instead.  All lanes of a group
 // reduce the error ratio and the collision gap to identical values, so the
 // step controller runs redundantly and stays consistent without a broadcast.
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "cbc_device.h"
 #include "cbc_kernels.h"
@@ -117,6 +119,22 @@ __device__ __forceinline__ void group_argmax(double& v, int& id)
     });
 }
 
+// Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
+// quad): a DPP quad_perm broadcast, no LDS.
+template <int G, int J>
+__device__ __forceinline__ double group_bcast(double v)
+{
+    static_assert(G == 2 || G == 4, "quad broadcast serves groups of 2 or 4 lanes");
+    constexpr int ctl = G == 4 ? J * 0x55 : (J | J << 2 | (2 + J) << 4 | (2 + J) << 6);
+    return dpp_double<ctl>(v);
+}
+
+template <int... J, typename F>
+__device__ __forceinline__ void for_lanes(std::integer_sequence<int, J...>, F f)
+{
+    (f(std::integral_constant<int, J>{}), ...);
+}
+
 // Per-lane constants of one bubble and its system.
 template <int NK>
 struct GroupParams {
@@ -178,17 +196,40 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         dy[3] = o.dV;
         return;
     } else {
-        if (vu) {
-            L.xh[0][i] = y[1];
-            L.xh[1][i] = o.H0;
-            L.xh[2][i] = o.H1;
-            L.xh[3][i] = o.H2;
-            L.xh[4][i] = o.H3;
-            L.xh[5][i] = o.H4;
-            L.xh[6][i] = o.H5;
-            L.xh[7][i] = o.H6;
+        // x, H0..H6 of every bubble of the group.  Pairs (N = 2) exchange them,
+        // and below the coupling rows, by DPP broadcasts inside the quad: the
+        // same values without an LDS round trip (an attempt 11.1 -> 10.3 us).
+        // Groups of 4 keep LDS: broadcasting their 72-entry system costs more
+        // than the LDS trip (N = 4 19.0 -> 20.8 us with both exchanges by DPP,
+        // 19.5 with this one alone).
+        constexpr int G = GroupShape<N>::G;
+        double xh[8][N];
+        if constexpr (G == 2) {
+            const double own[8] = {y[1], o.H0, o.H1, o.H2, o.H3, o.H4, o.H5, o.H6};
+            for_lanes(std::make_integer_sequence<int, N>{}, [&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) xh[c][j] = group_bcast<G, j>(own[c]);
+            });
+            // no LDS below for pairs (mx is written before the barrier that
+            // follows every system load), so no barrier either
+        } else {
+            if (vu) {
+                L.xh[0][i] = y[1];
+                L.xh[1][i] = o.H0;
+                L.xh[2][i] = o.H1;
+                L.xh[3][i] = o.H2;
+                L.xh[4][i] = o.H3;
+                L.xh[5][i] = o.H4;
+                L.xh[6][i] = o.H5;
+                L.xh[7][i] = o.H6;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int j = 0; j < N; ++j) xh[c][j] = L.xh[c][j];
         }
-        __syncthreads();
 
         // explicit coupling sums and the two matrix rows owned by this lane
         double ar[n + 1], at[n + 1];
@@ -196,7 +237,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         const int ii = vu ? i : 0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const double xj = L.xh[0][j];
+            const double xj = xh[0][j];
             const double delta = y[1] - xj;
             const double s = sgn(delta);
             const double r = rcp_fast(fmax(fabs(delta), 1e-30));
@@ -204,8 +245,8 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             const double sr2 = r2 * s;
             const double r3 = r2 * r;
             const double m0 = L.mx[0][ii][j], m1 = L.mx[1][ii][j], m2 = L.mx[2][ii][j];
-            const double H0 = L.xh[1][j], H1 = L.xh[2][j], H2 = L.xh[3][j], H3 = L.xh[4][j], H4 = L.xh[5][j];
-            const double H5 = L.xh[6][j], H6 = L.xh[7][j];
+            const double H0 = xh[1][j], H1 = xh[2][j], H2 = xh[3][j], H3 = xh[4][j], H4 = xh[5][j];
+            const double H5 = xh[6][j], H6 = xh[7][j];
             const bool self = (j == i);
             if (!self) {
                 rg0 += m0 * (-2.0 * r * H0 - 2.5 * sr2 * H2 - 1.0 * r3 * H4);
@@ -224,19 +265,30 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
 
         if constexpr (GroupShape<N>::redundant) {
             // stage the n rows, then every lane solves the whole system in VGPRs
-            if (vu) {
-#pragma unroll
-                for (int j = 0; j <= n; ++j) {
-                    L.rows[i][j] = ar[j];
-                    L.rows[N + i][j] = at[j];
-                }
-            }
-            __syncthreads();
             double a[n][n + 1];
+            if constexpr (G == 2) {
+                for_lanes(std::make_integer_sequence<int, N>{}, [&](auto jc) {
+                    constexpr int r = decltype(jc)::value;
 #pragma unroll
-            for (int r = 0; r < n; ++r)
+                    for (int j = 0; j <= n; ++j) {
+                        a[r][j] = group_bcast<G, r>(ar[j]);
+                        a[N + r][j] = group_bcast<G, r>(at[j]);
+                    }
+                });
+            } else {
+                if (vu) {
 #pragma unroll
-                for (int j = 0; j <= n; ++j) a[r][j] = L.rows[r][j];
+                    for (int j = 0; j <= n; ++j) {
+                        L.rows[i][j] = ar[j];
+                        L.rows[N + i][j] = at[j];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < n; ++r)
+#pragma unroll
+                    for (int j = 0; j <= n; ++j) a[r][j] = L.rows[r][j];
+            }
             double v[n];
             lu_solve<n>(a, v);
             double d2 = v[0], d3 = v[N];

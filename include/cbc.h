@@ -56,7 +56,7 @@ extern "C" {
 #define CBC_FIELD_SW_A 2
 
 /* system-to-hardware mapping */
-#define CBC_MAP_AUTO 0
+#define CBC_MAP_AUTO 0   /* N = 1 lane; N = 2 group while its waves fit one per SIMD, else lane; N >= 3 group */
 #define CBC_MAP_LANE 1   /* one system per lane, whole state in VGPRs */
 #define CBC_MAP_GROUP 2  /* TILE lanes per system, coupling solve staged in LDS */
 
