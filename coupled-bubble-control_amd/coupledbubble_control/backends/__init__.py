@@ -17,7 +17,7 @@ class BackendName(str, Enum):
 
 
 class KernelVariant(str, Enum):
-    AUTO = "auto"      # plan picks the mapping per bubble count
+    AUTO = "auto"      # plan picks the mapping per bubble count (for N = 2 also by ensemble size)
     LANE = "lane"      # one system per lane
     GROUP = "group"    # TILE lanes per system, LDS-staged coupling solve
     # reference names (backends/__init__.py:8-10), both resolved like AUTO
