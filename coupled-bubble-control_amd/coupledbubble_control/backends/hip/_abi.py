@@ -189,6 +189,7 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_uint32)]),
     "cbc_serve_destroy": (_i32, [_vp]),
     "cbc_selftest_math": (_i32, [_i32, _vp, _vp, _vp, _i32, _vp]),
+    "cbc_selftest_solve": (_i32, [_i32, _vp, _vp, _i32, _vp]),
     "cbc_env_workspace_words": (_i64, [_i32]),
     "cbc_env_set_action": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _vp]),
     "cbc_env_post": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _i64, _vp,

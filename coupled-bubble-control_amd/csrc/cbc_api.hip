@@ -16,6 +16,7 @@
 
 #include "../../include/cbc.h"
 #include "cbc_kernels.h"
+#include "cbc_linalg.h"
 #include "cbc_math.h"
 
 namespace {
@@ -87,6 +88,24 @@ __global__ void selftest_math(int fn, const double* x, const double* y, double* 
     }
     out[2 * (size_t)i] = u;
     out[2 * (size_t)i + 1] = v;
+}
+
+// The in-register coupling solve (lu_solve) for cbc_selftest_solve, one system
+// per thread.
+template <int n>
+__global__ void __launch_bounds__(64) selftest_solve(const double* a, double* x, int count)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= count) return;
+    const double* src = a + (size_t)s * n * (n + 1);
+    double m[n][n + 1], v[n];
+#pragma unroll
+    for (int r = 0; r < n; ++r)
+#pragma unroll
+        for (int j = 0; j <= n; ++j) m[r][j] = src[r * (n + 1) + j];
+    cbc::lu_solve<n>(m, v);
+#pragma unroll
+    for (int j = 0; j < n; ++j) x[(size_t)s * n + j] = v[j];
 }
 
 }  // namespace
@@ -435,6 +454,22 @@ int32_t cbc_selftest_math(int32_t fn, const double* x, const double* y, double* 
     void* args[] = {&f, &x, &y, &out, &n};
     hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&selftest_math), dim3((n + 255) / 256), dim3(256),
                                    args, 0, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "selftest kernel launch");
+    return CBC_OK;
+}
+
+int32_t cbc_selftest_solve(int32_t n, const double* a, double* x, int32_t count, void* stream)
+{
+    if (n != 2 && n != 4 && n != 6 && n != 8)
+        return fail(CBC_ERR_INVALID, "cbc_selftest_solve: n = %d not in {2, 4, 6, 8}", n);
+    if (count < 0 || (count > 0 && (!a || !x))) return fail(CBC_ERR_INVALID, "cbc_selftest_solve: bad arrays");
+    if (count == 0) return CBC_OK;
+    const void* k = n == 2 ? reinterpret_cast<const void*>(&selftest_solve<2>)
+                  : n == 4 ? reinterpret_cast<const void*>(&selftest_solve<4>)
+                  : n == 6 ? reinterpret_cast<const void*>(&selftest_solve<6>)
+                           : reinterpret_cast<const void*>(&selftest_solve<8>);
+    void* args[] = {&a, &x, &count};
+    hipError_t e = hipLaunchKernel(k, dim3((count + 63) / 64), dim3(64), args, 0, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "selftest kernel launch");
     return CBC_OK;
 }

@@ -543,6 +543,13 @@ int32_t cbc_traj_gather(const cbc_traj_spec *spec, const cbc_traj_buffers *buf, 
 #define CBC_MATH_POW1P 8
 int32_t cbc_selftest_math(int32_t fn, const double *x, const double *y, double *out, int32_t n, void *stream);
 
+/* The kernels' in-register coupling solve (LU with partial pivoting by conditional row
+ * exchanges) on `count` augmented systems a (count x n x (n + 1), row major), n in
+ * {2, 4, 6, 8} (the 2N unknowns for N = 1..4); solutions into x (count x n).  No
+ * counterpart in the reference (which calls a library solve); it lets tests drive every
+ * exchange path of the solve on chosen matrices.  Asynchronous. */
+int32_t cbc_selftest_solve(int32_t n, const double *a, double *x, int32_t count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

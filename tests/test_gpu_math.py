@@ -83,3 +83,56 @@ def test_stage_reuse_forms():
     got = _eval("pow1p", e, y)[0]
     want = np.exp(y * np.log1p(e))
     assert np.max(np.abs(got / want - 1.0)) <= 4e-16
+
+
+def _solve(n, a):
+    import torch
+
+    from coupledbubble_control.backends.hip import _abi
+
+    da = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    x = torch.empty((a.shape[0], n), dtype=torch.float64, device="cuda")
+    _abi.check(_abi.lib().cbc_selftest_solve(n, da.data_ptr(), x.data_ptr(), a.shape[0], None))
+    torch.cuda.synchronize()
+    return x.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 8])
+def test_coupling_solve_every_exchange_path(n):
+    """The in-register LU on systems that need no exchange (column-diagonally dominant
+    I + C: partial pivoting keeps the diagonal), an exchange with the next row only, an
+    exchange with a row further down at nearly every step (the dominant system with its
+    rows shuffled, so each column's pivot sits at a random row below), tiny diagonals
+    (an elimination without the right pivots loses most digits there), a first pivot two
+    or more rows down, and general systems; solutions against LAPACK's
+    (numpy.linalg.solve)."""
+    print(f"case: coupling solve n={n}", flush=True)
+    rng = np.random.default_rng(70 + n)
+    m = 512
+    c = rng.uniform(-0.9, 0.9, (m, n, n)) / max(n - 1, 1)
+    dom = np.concatenate([c, rng.uniform(-1, 1, (m, n, 1))], axis=2)
+    dom[:, np.arange(n), np.arange(n)] = 1.0
+    nxt = dom.copy()
+    nxt[:, [0, 1]] = nxt[:, [1, 0]]
+    shuf = np.stack([s[rng.permutation(n)] for s in dom])
+    tiny = rng.uniform(-1, 1, (m, n, n + 1))
+    tiny[:, np.arange(n), np.arange(n)] *= 1e-10
+    gen = rng.uniform(-1, 1, (m, n, n + 1))
+    cases = [("dominant", dom), ("next row", nxt), ("shuffled rows", shuf), ("tiny diagonal", tiny), ("general", gen)]
+    if n > 2:
+        # tiny diagonal and first sub-diagonal: the first pivot lies two or more rows
+        # down; an elimination that exchanged with the next row at most loses 9-11
+        # digits here (host check), the full rule none
+        far = rng.uniform(-1, 1, (m, n, n + 1))
+        far[:, np.arange(n), np.arange(n)] *= 1e-10
+        far[:, np.arange(1, n), np.arange(n - 1)] *= 1e-10
+        cases.append(("far pivot", far))
+    for name, a in cases:
+        A, b = a[:, :, :n], a[:, :, n]
+        want = np.linalg.solve(A, b[..., None])[..., 0]
+        got = _solve(n, a)
+        cond = np.linalg.cond(A)
+        err = np.abs(got - want).max(axis=1) / np.abs(want).max(axis=1)
+        ok = cond < 1e8
+        assert ok.sum() > m // 2, name
+        assert np.all(err[ok] <= 1e-14 * n * cond[ok] + 1e-15), (name, float(err[ok].max()))
