@@ -92,3 +92,21 @@ def test_group_dense_and_windows(device):
     w3 = _mixed(3, 32, 1.0)
     m3, ref3, _ = _run_pair(w3, ndo=8, variant=KernelVariant.GROUP)
     np.testing.assert_array_equal(m3.runtime.host_buffers.dense.dense_index, ref3["dense_index"])
+
+
+def test_auto_mapping_for_two_bubbles(device):
+    """AUTO runs N = 2 in lane pairs while their waves fit one per SIMD (up to 32 768
+    systems on 256 CUs) and one system per lane beyond."""
+    import torch
+
+    from coupledbubble_control import workloads
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    limit = 4 * cus * 64 // 2
+    for ns, want in ((4096, "group"), (limit, "group"), (limit + 32, "lane"), (65536, "lane")):
+        print(f"case: AUTO mapping, N=2, {ns} systems", flush=True)
+        w = workloads.plain_cpu(seed=3, ns=ns)
+        w.t_end[:] = 0.01
+        m = workloads.build_model(w)
+        m.solve(max_steps=1 << 20, kernel_steps=1 << 20)
+        assert m.runtime.kernel.info().mapping == want, (ns, cus)
