@@ -22,7 +22,10 @@ bubbles with implicit radial/translational coupling):
 * ``param_study`` -- 65 536 systems of N=2 on a 256 x 256 (PA1, PA2) grid over
   [0, 2] bar, R0 = (44, 60) um, 200 periods with per-period max R/R0 and mean x
   for periods 51..200, rtol = atol = 1e-10; a step is ``--periods-per-step``
-  more periods of every system (one launch of the window kernel).
+  more periods of every system (one launch of the window kernel), by default the
+  whole 200-period job in one step, as the reference integrates the horizon in
+  one solver call (a launch lasts as long as its slowest system, so 20 steps of
+  10 periods add the per-launch tails: 3.09e6 against 3.43e6 periods/s).
 * ``mixed`` -- 32 768 systems with N in {2, 3, 4, 8} at 8:4:2:1 and horizons of
   1..50 periods, rtol = atol = 1e-9: the autotuned persistent work-queue
   ensemble against one plain launch loop per bubble count; a step is the whole
@@ -82,11 +85,13 @@ def parse():
     p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env"], default="serving")
     p.add_argument("--steps", type=int, default=None,
                    help="serving: requests timed (default 1100, ~3 s of arrivals); param_study: steps of "
-                        "--periods-per-step (default 20); mixed/large: ensemble solves (default 1)")
+                        "--periods-per-step (default: as many as cover 200 periods); mixed/large: ensemble "
+                        "solves (default 1)")
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--rate", type=float, default=200000.0, help="serving: offered env-steps/s")
     p.add_argument("--saturation-s", type=float, default=3.0, help="serving: closed-loop seconds per path (0: off)")
-    p.add_argument("--periods-per-step", type=float, default=10.0)
+    p.add_argument("--periods-per-step", type=float, default=200.0,
+                   help="param_study: periods integrated per step (one window-kernel launch)")
     p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
     p.add_argument("--env-bubbles", type=int, default=2, help="env: bubbles per env")
     p.add_argument("--env-count", type=int, default=16384, help="env: number of envs")
@@ -270,8 +275,8 @@ def bench_param_study(args):
 
     dev = torch.device("cuda:0")
     P = float(args.periods_per_step)
-    steps = args.steps or 20
     total_periods = 200.0
+    steps = args.steps or max(1, int(round(total_periods / P)))
     w = W.param_study(side=args.side, periods=total_periods, discard=50)
     ns = w.ns
     F = flops_per_attempt(w.n)
