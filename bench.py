@@ -48,6 +48,7 @@ same workload.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -597,6 +598,8 @@ def bench_env(args):
     a0 = attempts_of([env.model])[n]
     lat, dones = [], 0
     log(f"env: {steps} timed steps")
+    gc.collect()
+    gc.disable()   # no collector pass inside a timed step (as timeit does)
     t0 = time.perf_counter()
     for k in range(steps):
         s0 = time.perf_counter()
@@ -605,6 +608,7 @@ def bench_env(args):
         lat.append(time.perf_counter() - s0)
         dones += int(info["dones"].numel()) if info else 0
     wall = time.perf_counter() - t0
+    gc.enable()
     att = attempts_of([env.model])[n] - a0
     flush_s = 0.0
     if traj:
