@@ -193,6 +193,8 @@ def bench_serving(args):
     def run(mode, reqs=None, closed=None, per_bin=False):
         eng, bins = engine(mode, per_bin)
         a0 = attempts_of(bins.values())
+        gc.collect()
+        gc.disable()   # no collector pass inside the timed request loop (as timeit does)
         try:
             if closed is not None:
                 first, feeder = closed_loop(venvs, closed, seed=9)
@@ -200,6 +202,7 @@ def bench_serving(args):
             else:
                 st = eng.run(reqs, timeout_s=300.0, progress=log)
         finally:
+            gc.enable()
             eng.close()
         a1 = attempts_of(bins.values())
         st["attempts"] = {n: a1[n] - a0[n] for n in a1}
