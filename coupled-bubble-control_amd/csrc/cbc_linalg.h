@@ -30,11 +30,14 @@ __device__ __forceinline__ void lu_cascade(double (&a)[n][n + 1], int k, int fro
 // largest |a_ik| ends up as pivot, as in the CPU restatement.
 //
 // Exchanges are not rare in this model, but they follow a pattern: on the
-// serving, mixed and plain ensembles (oracle LU, 69 000-142 000 solves per set)
-// an exchange happens only in the first N - 1 (radial) columns, for 1-7 % of
-// the solves per column, and always with the next row.  With 16-64 systems
-// per wave the exchange code of those columns runs in a third to two thirds of
-// the waves.  For n = 8 (the group mapping at N = 4), where some lane of the
+// serving, mixed and plain ensembles (oracle LU, 69 000-1 300 000 solves per
+// set) an exchange happens only in the first N - 1 (radial) columns, for 1-7 %
+// of the solves per column, and with the next row in all but ~0.1 % of the
+// solves (a 512-system serving-style set of 4 bubbles: 0.1 % at the second
+// column take a row two down).  With 16-64 systems per wave the exchange code
+// of those columns runs in a third to two thirds of the waves.  The rare far
+// exchanges stay in registers too: solving those systems again from their
+// staged rows, serially in LDS, made an N = 4 attempt 19.3 -> 27.8 us.  For n = 8 (the group mapping at N = 4), where some lane of the
 // wave needs an exchange at a step, the cascade runs as its first exchange (row
 // k + 1, n + 1 - k selects per row) and then the rest (rows k + 2.., tested
 // against the diagonal after the first) only if some lane needs that: the same
