@@ -60,10 +60,12 @@ sys.path.insert(0, str(ROOT / "coupled-bubble-control_amd"))
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
-# HBM bytes per system per 10-period launch of the parameter-study kernel, measured with
-# rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a copy kernel of known
-# size): 528 B read + 100 B written (profiles/lane_kernel_param_study_65536xN2.md)
-PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM = 628
+# HBM bytes per system of one launch of the parameter-study kernel over the whole 200-period
+# job, measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a copy
+# kernel of known size): 539 B read + 3 416 B written, the writes mostly the per-period window
+# statistics (profiles/lane_kernel_param_study_65536xN2.md).  Only that launch length was
+# measured; other --periods-per-step report no traffic.
+PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
@@ -363,8 +365,10 @@ def bench_param_study(args):
         "roofline": roofline(attempts * F / steps, float(np.mean(kernel_ms)) * 1e-3,
                              window="kernel (HIP events per launch)", flops_per_attempt=F, attempts=int(attempts),
                              kernel_ms_mean=float(np.mean(kernel_ms)),
-                             traffic=PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM * ns,
-                             traffic_note="HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE"),
+                             traffic=(PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 * ns
+                                      if P == total_periods and steps == 1 and ns == 65536 else None),
+                             traffic_note="HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE of the "
+                                          "200-period launch of 65 536 systems"),
         "cpu_baseline": cpu,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(flags, return_counts=True))},
     }
