@@ -52,29 +52,59 @@ class ServedVecEnv:
         observed positions (num_envs, N) as float32 and the failure mask."""
         import time
 
-        pa = np.ascontiguousarray(np.asarray(pa_bar, dtype=np.float64).reshape(self.num_envs, self.nk).T) * 1.0e5
-        h0 = np.where(self._reset, 1e-6, 0.0 if self.keep_step else 1e-6)
-        seq = self.queue.submit(self._ids, pa, self.T, h0, self._reset if self._reset.any() else None)
+        seq = self.submit(self._ids, pa_bar)
         if seq is None:
             raise RuntimeError("serving ring full")
-        self._reset[:] = False
         t0 = time.time()
-        while not self.queue.done(self._ids, seq).all():
+        while not self.finished(self._ids, seq):
             self.queue.poll()
             if time.time() - t0 > self.timeout_s:
                 raise TimeoutError("env step did not finish")
             time.sleep(1e-4)
-        state, misc = self.queue.results(self._ids)
-        self.observations = state[1].astype(np.float32)
-        self.failures = misc[:, 2] != 1.0
-        if self.scene is not None and self.failures.any():
-            self.reset_envs(np.nonzero(self.failures)[0])
+        self.collect(self._ids)
         return self.observations, self.failures
 
-    def reset_envs(self, env_ids) -> None:
+    def submit(self, env_ids: np.ndarray, pa_bar: np.ndarray) -> Optional[np.ndarray]:
+        """Publish one step of ``env_ids`` with amplitudes ``pa_bar`` (len(env_ids), NK) [bar]
+        without waiting; returns the items' sequence numbers, or None when the ring has too
+        few free slots (nothing published: poll and try again)."""
+        ids = np.ascontiguousarray(env_ids, dtype=np.int32)
+        pa = np.ascontiguousarray(np.asarray(pa_bar, dtype=np.float64).reshape(ids.size, self.nk).T) * 1.0e5
+        rs = self._reset[ids]
+        h0 = np.where(rs, 1e-6, 0.0 if self.keep_step else 1e-6)
+        seq = self.queue.submit(ids, pa, self.T, h0, rs if rs.any() else None)
+        if seq is not None:
+            self._reset[ids] = False
+        return seq
+
+    def finished(self, env_ids: np.ndarray, seq: np.ndarray) -> bool:
+        """Whether the step with sequence numbers ``seq`` of ``env_ids`` has finished."""
+        return bool(self.queue.done(env_ids, seq).all())
+
+    def collect(self, env_ids: np.ndarray, rng: Optional[np.random.Generator] = None
+                ) -> Tuple[np.ndarray, np.ndarray]:
+        """Observations (len(env_ids), N) and failures of the finished step of ``env_ids``
+        (also written into ``observations`` / ``failures``); failed envs get new scenes
+        for their next step, drawn from ``rng`` (default: the env's own generator)."""
+        ids = np.asarray(env_ids)
+        state, misc = self.queue.results(ids)
+        obs = state[1].astype(np.float32)
+        fail = misc[:, 2] != 1.0
+        # new arrays each time: observations returned by earlier steps stay as they were
+        prev_o = self.observations if self.observations is not None else \
+            np.zeros((self.num_envs, self.num_bubbles), dtype=np.float32)
+        prev_f = self.failures if self.failures is not None else np.zeros(self.num_envs, dtype=bool)
+        self.observations, self.failures = prev_o.copy(), prev_f.copy()
+        self.observations[ids] = obs
+        self.failures[ids] = fail
+        if self.scene is not None and fail.any():
+            self.reset_envs(ids[fail], rng)
+        return obs, fail
+
+    def reset_envs(self, env_ids, rng: Optional[np.random.Generator] = None) -> None:
         """New positions for ``env_ids`` (radii kept), taken by their next step."""
         env_ids = np.asarray(env_ids, dtype=np.int64)
-        X0 = self.scene(env_ids.size, self.rng)
+        X0 = self.scene(env_ids.size, self.rng if rng is None else rng)
         state, up, cm = self.model.scene_rows(env_ids, None, X0)
         self.queue.stage(env_ids, state, up, cm)
         self._reset[env_ids] = True

@@ -317,3 +317,41 @@ def test_served_vec_env_equals_blocking_env_step():
     finally:
         env.close()
     assert total_fail > 0   # the reset path was exercised
+
+
+def test_overlapped_rollout_equals_synchronous():
+    """Four rollout steps of 256 envs (N = 2) under a deterministic policy: the envs split
+    into three groups that step on their own see the observations, actions and failures
+    of one group and of the synchronous ServedVecEnv.step loop, bit for bit."""
+    from coupledbubble_control.envs import ServedVecEnv
+    from coupledbubble_control.rollout import OverlappedRollout
+
+    def policy(o, ids):
+        x = o.astype(np.float64)
+        return np.clip(0.75 + 0.6 * np.sin(40.0 * x[:, :2] + 0.01 * ids[:, None]), 0.0, 1.5)
+
+    steps = 4
+    got = {}
+    for groups in (0, 1, 3):   # 0: the synchronous loop
+        print(f"\n[rollout] N=2 ns=256, {steps} steps, groups {groups}", flush=True)
+        w, m, _ = _models(2, 256, seed=31)
+        obs0 = m.runtime.host_buffers.state.actual_state[1, :, :2].astype(np.float32)
+        env = ServedVecEnv(m, time_step_length=2.0, scene=None)
+        try:
+            if groups == 0:
+                ids = np.arange(256)
+                obs, acts, fails = [obs0], [], []
+                for _ in range(steps):
+                    a = policy(obs[-1], ids)
+                    o, f = env.step(a)
+                    obs.append(o)
+                    acts.append(a)
+                    fails.append(f)
+                got[groups] = {"observations": np.stack(obs), "actions": np.stack(acts), "failures": np.stack(fails)}
+            else:
+                got[groups] = OverlappedRollout(env, policy, groups=groups).collect(obs0, steps)
+        finally:
+            env.close()
+    for groups in (1, 3):
+        for k in ("observations", "actions", "failures"):
+            np.testing.assert_array_equal(got[groups][k], got[0][k], err_msg=f"groups {groups}: {k}")
