@@ -38,6 +38,14 @@ bubbles with implicit radial/translational coupling):
   scenes), random actions; a step is one env.step (set_action, solve, observe,
   termination, rewards, advance_time, resets of finished episodes).  Reports
   step latency and the split between the solver and the rest of the step.
+* ``rollout`` -- PPO-style rollout collection over the serving queue: 16 384
+  resident envs of 2 bubbles (``--env-count``), 5 periods per step, failed envs
+  restarted with new positions, actions from an actor MLP (256x256 tanh, Beta
+  mean) on each env's observed positions.  The synchronous loop (every env steps,
+  then the policy runs on every env, as rl/ppo.py:249-300 does) against
+  ``--rollout-groups`` groups of envs stepping on their own with the policy run
+  on each finished group (rollout.py); value = the grouped loop's nominal
+  periods/s (env steps x 5 periods), a step = one rollout step of every env.
 
 Every line carries the FP64 roofline of the dominant kernel (algorithmic FLOPs
 per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
@@ -85,7 +93,8 @@ def flops_per_attempt(n: int) -> int:
 def parse():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env"], default="serving")
+    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env", "rollout"],
+                   default="serving")
     p.add_argument("--steps", type=int, default=None,
                    help="serving: requests timed (default 1100, ~3 s of arrivals); param_study: steps of "
                         "--periods-per-step (default: as many as cover 200 periods); mixed/large: ensemble "
@@ -98,6 +107,7 @@ def parse():
     p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
     p.add_argument("--env-bubbles", type=int, default=2, help="env: bubbles per env")
     p.add_argument("--env-count", type=int, default=16384, help="env: number of envs")
+    p.add_argument("--rollout-groups", type=int, default=4, help="rollout: groups of envs stepping on their own")
     p.add_argument("--env-trajectories", type=int, default=0,
                    help="env: record trajectories (dense output, device arena, zarr files in a temporary "
                         "directory); 2 writes the files on a background thread")
@@ -661,6 +671,92 @@ def bench_env(args):
     return out
 
 
+def bench_rollout(args):
+    import numpy as np
+    import torch
+
+    from coupledbubble_control import workloads as W
+    from coupledbubble_control.envs import ServedVecEnv
+    from coupledbubble_control.rollout import OverlappedRollout
+
+    n, nk, K = 2, 2, 5.0
+    ne = args.env_count
+    steps = args.steps or 20
+    lam = W.wavelength(W.water_25c(), 25.0)
+    torch.manual_seed(0)
+
+    def mlp(out):
+        return torch.nn.Sequential(torch.nn.Linear(n, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
+                                   torch.nn.Tanh(), torch.nn.Linear(256, out)).cuda()
+    actor = mlp(2 * nk)
+    policy_s = [0.0]
+
+    def policy(o, ids):
+        c0 = time.perf_counter()
+        with torch.no_grad():
+            ab = torch.nn.functional.softplus(actor(torch.from_numpy(o).cuda())) + 1.0
+            pa = (1.5 * ab[:, :nk] / (ab[:, :nk] + ab[:, nk:])).double().cpu().numpy()
+        policy_s[0] += time.perf_counter() - c0
+        return pa
+
+    def run(groups):
+        w = W.serving_bins(seed=3, sizes={n: ne}, periods=K)[n]
+        gap = np.full(1, 3.0 * float(w.R0.max()) * 1e-6 / lam)
+
+        def scene(k, rng):
+            return W.sample_positions(rng, k, n, 0.0, 0.25, np.repeat(gap, k))
+        m = W.build_model(w)
+        env = ServedVecEnv(m, time_step_length=K, scene=scene, seed=5)
+        ro = OverlappedRollout(env, policy, groups=groups, seed=7)
+        try:
+            obs0 = m.runtime.host_buffers.state.actual_state[1, :, :n].astype(np.float32)
+            ro.collect(obs0, max(args.warmup, 1))
+            obs1 = env.observations.copy()
+            a0 = attempts_of([m])[n]
+            policy_s[0] = 0.0
+            gc.collect()
+            gc.disable()   # no collector pass inside the timed loop (as timeit does)
+            t0 = time.perf_counter()
+            r = ro.collect(obs1, steps)
+            wall = time.perf_counter() - t0
+            gc.enable()
+            att = attempts_of([m])[n] - a0
+        finally:
+            env.close()
+        return {"wall_s": wall, "attempts": att, "policy_s": policy_s[0],
+                "failures": int(r["failures"].sum()), "group_done_s": [round(x, 4) for x in r["group_done_s"]]}
+
+    G = args.rollout_groups
+    log(f"rollout: {ne} envs x {steps} steps, synchronous loop")
+    sync = run(1)
+    log(f"rollout: {G} groups")
+    grp = run(G)
+    periods = ne * steps * K
+    cpu = None
+    if args.cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_step_sample(W, [W.serving_bins(seed=3, sizes={n: ne}, periods=K)[n]], args.cpu_seconds,
+                              "the resident envs, one 5-period step each")
+    F = flops_per_attempt(n)
+    return {
+        "value": periods / grp["wall_s"],
+        "unit": "periods/s",
+        "steps": steps,
+        "ms_per_step": grp["wall_s"] * 1e3 / steps,
+        "config": {"workload": f"rollout: {ne} served envs x {n} bubbles, {K:g} periods per step, actor MLP "
+                               f"policy, {G} groups", "envs": ne, "bubbles": n, "periods_per_step": K,
+                   "groups": G, "policy": "actor MLP 256x256 tanh, Beta mean, on the observed positions",
+                   "periods": "nominal: env steps x 5 (failed envs restart at the next step)"},
+        "env_steps_per_s": ne * steps / grp["wall_s"],
+        "synchronous": {**sync, "periods_per_s": periods / sync["wall_s"], "ms_per_step": sync["wall_s"] * 1e3 / steps},
+        "grouped": {**grp, "periods_per_s": periods / grp["wall_s"]},
+        "vs_synchronous": sync["wall_s"] / grp["wall_s"],
+        "roofline": roofline(grp["attempts"] * F, grp["wall_s"], window="timed region (persistent serving kernel)",
+                             flops_per_attempt=F, attempts=int(grp["attempts"])),
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -677,6 +773,8 @@ def main():
         out = bench_param_study(args)
     elif args.workload == "env":
         out = bench_env(args)
+    elif args.workload == "rollout":
+        out = bench_rollout(args)
     else:
         out = bench_ensemble(args, large=args.workload == "large")
     line = {"metric": METRIC, "value": out.pop("value"), "unit": out.pop("unit"), "n_gpus": 1,
