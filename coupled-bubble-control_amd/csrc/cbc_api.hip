@@ -203,9 +203,6 @@ int32_t cbc_plan_create(const cbc_model_spec* model, const cbc_exec_spec* exec, 
         } else {
             return fail(CBC_ERR_INVALID, "unknown mapping %d", mapping);
         }
-        if (solver->event_mode == CBC_EVENT_EXACT && mapping == CBC_MAP_GROUP && exec->ups > 4)
-            return fail(CBC_ERR_UNSUPPORTED, "the exact event mode is built for up to 4 bubbles per system, got %d",
-                        exec->ups);
 
         cbc_plan* p = new (std::nothrow) cbc_plan();
         if (!p) return fail(CBC_ERR_NOMEM, "cbc_plan_create: out of host memory");
@@ -243,7 +240,8 @@ int32_t cbc_plan_query(const cbc_plan* plan, cbc_plan_info* out)
     cbc::KernelAttr at{};
     const bool lane = plan->mapping == CBC_MAP_LANE;
     hipError_t e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                        : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+                        : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
+                                          plan->solver.event_mode == CBC_EVENT_EXACT, &at);
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
     const int spb = lane ? plan->block : cbc::group_systems_per_block(plan->exec.ups);
     out->mapping = plan->mapping;
@@ -408,7 +406,8 @@ static int32_t queued_args(cbc_plan* plan, const cbc_buffers* buf, const int32_t
         if (e != hipSuccess) return hip_fail(e, "device query");
         cbc::KernelAttr at{};
         e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
+                                   plan->solver.event_mode == CBC_EVENT_EXACT, &at);
         if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
         // waves resident per SIMD from the register budget (512 per lane on gfx950)
         const int regs = at.num_regs > 0 ? at.num_regs : 512;
@@ -679,7 +678,8 @@ int32_t cbc_serve_create(cbc_plan* plan, const cbc_buffers* buf, const cbc_serve
         const bool lane = plan->mapping == CBC_MAP_LANE;
         cbc::KernelAttr at{};
         e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
+                                   plan->solver.event_mode == CBC_EVENT_EXACT, &at);
         if (e != hipSuccess) {
             serve_free(s);
             return hip_fail(e, "hipFuncGetAttributes");

@@ -167,7 +167,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
 // One attempt for every group of the wave (uniform: contains barriers).  `busy`
 // marks groups with a live system; returns true for a busy group whose system
 // stopped this attempt.
-template <int N, int NK, int FIELD>
+template <int N, int NK, int FIELD, bool EXACT>
 __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupLds<N>& L, GroupRun<N, NK>& R,
                                               bool busy)
 {
@@ -251,9 +251,9 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     }
     const double ratio = group_min<G>(ratio_u);
     const bool bad = group_any<G>(bad_u);
-    // exact event mode (N <= 4 only: the Gauss--Jordan kernels for N = 8, 16 are
-    // built without it): the gap's root on the step, for groups that cross
-    constexpr bool kExact = GroupShape<N>::redundant;
+    // exact event mode: the gap's root on the step, for groups that cross (the
+    // Gauss--Jordan kernels for N = 8, 16 come in a bracket-only and an exact build)
+    constexpr bool kExact = EXACT;
     double theta = 1.0;
     if constexpr (kExact) {
         if (a.event_exact) {
@@ -378,7 +378,7 @@ __device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const G
     }
 }
 
-template <int N, int NK, int FIELD>
+template <int N, int NK, int FIELD, bool EXACT>
 __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
 {
     using Sh = GroupShape<N>;
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
         }
         idle_since = 0;
         ++since_poll;
-        const bool stop = group_attempt<N, NK, FIELD>(a, i, L, R, busy);
+        const bool stop = group_attempt<N, NK, FIELD, EXACT>(a, i, L, R, busy);
         if (busy && (stop || ++used >= a.kernel_steps)) {
             group_finish<N, NK>(a, i, R);
             busy = false;
@@ -495,10 +495,18 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
     }
 }
 
+// The kernels with the redundant solve (N <= 4) carry the exact event block in
+// every build and test the mode at run time.  Compiled into the Gauss--Jordan
+// kernels (N = 8, 16) that block made bracket-mode runs return NaN, so those
+// bubble counts have a bracket-only build, used for the bracket mode, and an
+// exact build, used only for the exact mode.
 template <int N>
-const void* group_pick(int nk, int field)
+const void* group_pick(int nk, int field, bool exact)
 {
-#define CBC_G(NK_, F_) return reinterpret_cast<const void*>(&group_kernel<N, NK_, F_>)
+    constexpr bool kAlways = GroupShape<N>::redundant;
+#define CBC_G(NK_, F_)                                                                   \
+    return exact || kAlways ? reinterpret_cast<const void*>(&group_kernel<N, NK_, F_, true>) \
+                            : reinterpret_cast<const void*>(&group_kernel<N, NK_, F_, kAlways>)
     if (nk == 1) {
         if (field == kFieldConst) CBC_G(1, kFieldConst);
         if (field == kFieldSwN) CBC_G(1, kFieldSwN);

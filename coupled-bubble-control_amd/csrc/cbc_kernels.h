@@ -109,7 +109,7 @@ bool lane_supported(int ups, int nk, int field);
 // Group mapping: next_pow2(ups) lanes per system, one wave per workgroup.
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
 hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream);
-hipError_t group_attr(int ups, int nk, int field, KernelAttr* out);
+hipError_t group_attr(int ups, int nk, int field, bool exact, KernelAttr* out);
 bool group_supported(int ups, int nk, int field);
 int group_systems_per_block(int ups);
 

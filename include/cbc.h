@@ -85,9 +85,8 @@ typedef struct cbc_exec_spec {
  *   retried with h * theta, theta the root of the gap along the step's cubic
  *   Hermite interpolant of R and x (slopes U and V, the state's own derivatives),
  *   and a step that crosses with |g_new| < etol is accepted; with etol ~1e-12
- *   the run ends at the root to that tolerance.  Built for the lane mapping and
- *   for the group mapping up to 4 bubbles; cbc_plan_create returns
- *   CBC_ERR_UNSUPPORTED for EXACT with more bubbles. */
+ *   the run ends at the root to that tolerance.  Built for both mappings and
+ *   every bubble count (for 8 and 16 bubbles per system in kernels of its own). */
 #define CBC_EVENT_BRACKET 0
 #define CBC_EVENT_EXACT 1
 

@@ -63,8 +63,7 @@ def test_plan_create_rejects_bad_specs_without_gpu_work():
 
 
 def test_plan_create_rejects_bad_event_modes_without_gpu_work():
-    """An unknown event mode is invalid; the exact mode is built for up to 4
-    bubbles per system in the group mapping."""
+    """An unknown event mode is invalid."""
     from coupledbubble_control.backends.hip import _abi
 
     lib = _abi.lib()
@@ -75,8 +74,3 @@ def test_plan_create_rejects_bad_event_modes_without_gpu_work():
     assert lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h)) == \
         _abi.CBC_ERR_INVALID
     assert b"event mode" in lib.cbc_last_error()
-    e = _abi.ExecSpecC(16, 1, 8, 0, 8, _abi.CBC_MAP_AUTO, 0)
-    s.event_mode = _abi.CBC_EVENT_EXACT
-    assert lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h)) == \
-        _abi.CBC_ERR_UNSUPPORTED
-    assert b"up to 4 bubbles" in lib.cbc_last_error()
