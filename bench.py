@@ -74,6 +74,14 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
 # statistics (profiles/lane_kernel_param_study_65536xN2.md).  Only that launch length was
 # measured; other --periods-per-step report no traffic.
 PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
+# Fabric bytes of the two persistent serving kernels over the timed region of the default
+# serving run (1 100 requests at 2e5 env-steps/s), the same counters and factors: 4.33 GB read +
+# 0.36 GB written (7 914 + 649 B per env step).  The writes scale with the env steps; the reads
+# are mostly the waves' polls of the request ring and scale with the time the region lasts (7.7 s
+# under counter collection, ~5.6 s without).  Ring, control words and outputs live in mapped
+# host memory, so this bounds the HBM bytes from above.  Other request counts or rates report
+# no traffic.
+SERVING_TRAFFIC_BYTES_1100 = 4682015437
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
@@ -276,7 +284,12 @@ def bench_serving(args):
         "saturation": sat,
         "roofline": roofline(cont["flops"], cont["wall_s"], window="timed region (persistent serving kernels)",
                              flops_per_attempt={str(n): flops_per_attempt(n) for n in sizes},
-                             attempts={str(n): v for n, v in cont["attempts"].items()}),
+                             attempts={str(n): v for n, v in cont["attempts"].items()},
+                             traffic=(SERVING_TRAFFIC_BYTES_1100 if steps == 1100 and args.rate == 200000.0
+                                      else None),
+                             traffic_note="bytes of both serving kernels over the timed region from rocprofv3 "
+                                          "FETCH_SIZE/WRITE_SIZE of the default run, mostly ring polls to mapped "
+                                          "host memory (an upper bound on HBM bytes)"),
         "cpu_baseline": cpu,
     }
 
