@@ -44,8 +44,10 @@ bubbles with implicit radial/translational coupling):
   mean) on each env's observed positions.  The synchronous loop (every env steps,
   then the policy runs on every env, as rl/ppo.py:249-300 does) against
   ``--rollout-groups`` groups of envs stepping on their own with the policy run
-  on each finished group (rollout.py); value = the grouped loop's nominal
-  periods/s (env steps x 5 periods), a step = one rollout step of every env.
+  on each finished group (value: the grouped loop's nominal periods/s, env steps
+  x 5 periods), and every env stepping on its own, the policy run on batches of
+  >= 1/16 of the envs whose steps finished (``per_env``; rollout.py); a step =
+  one rollout step of every env.
 
 Every line carries the FP64 roofline of the dominant kernel (algorithmic FLOPs
 per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
@@ -703,8 +705,10 @@ def bench_rollout(args):
                                    torch.nn.Tanh(), torch.nn.Linear(256, out)).cuda()
     actor = mlp(2 * nk)
     policy_s = [0.0]
+    calls = [0]
 
     def policy(o, ids):
+        calls[0] += 1
         c0 = time.perf_counter()
         with torch.no_grad():
             ab = torch.nn.functional.softplus(actor(torch.from_numpy(o).cuda())) + 1.0
@@ -723,10 +727,10 @@ def bench_rollout(args):
         ro = OverlappedRollout(env, policy, groups=groups, seed=7)
         try:
             obs0 = m.runtime.host_buffers.state.actual_state[1, :, :n].astype(np.float32)
-            ro.collect(obs0, max(args.warmup, 1))
-            obs1 = env.observations.copy()
+            obs1 = ro.collect(obs0, max(args.warmup, 1))["observations"][-1].copy()
             a0 = attempts_of([m])[n]
             policy_s[0] = 0.0
+            calls[0] = 0
             gc.collect()
             gc.disable()   # no collector pass inside the timed loop (as timeit does)
             t0 = time.perf_counter()
@@ -736,14 +740,20 @@ def bench_rollout(args):
             att = attempts_of([m])[n] - a0
         finally:
             env.close()
-        return {"wall_s": wall, "attempts": att, "policy_s": policy_s[0],
-                "failures": int(r["failures"].sum()), "group_done_s": [round(x, 4) for x in r["group_done_s"]]}
+        out = {"wall_s": wall, "attempts": att, "policy_s": policy_s[0], "policy_calls": calls[0],
+               "failures": int(r["failures"].sum()), "group_done_s": [round(float(x), 4) for x in r["group_done_s"]]}
+        if "env_done_s" in r:
+            out["env_done_s_p50_max"] = [round(float(np.median(r["env_done_s"])), 4),
+                                         round(float(r["env_done_s"].max()), 4)]
+        return out
 
     G = args.rollout_groups
     log(f"rollout: {ne} envs x {steps} steps, synchronous loop")
     sync = run(1)
     log(f"rollout: {G} groups")
     grp = run(G)
+    log("rollout: every env on its own")
+    per = run("env")
     periods = ne * steps * K
     cpu = None
     if args.cpu_baseline:
@@ -763,6 +773,8 @@ def bench_rollout(args):
         "env_steps_per_s": ne * steps / grp["wall_s"],
         "synchronous": {**sync, "periods_per_s": periods / sync["wall_s"], "ms_per_step": sync["wall_s"] * 1e3 / steps},
         "grouped": {**grp, "periods_per_s": periods / grp["wall_s"]},
+        "per_env": {**per, "policy_min_batch": max(1, ne // 16), "periods_per_s": periods / per["wall_s"],
+                    "ms_per_step": per["wall_s"] * 1e3 / steps, "vs_synchronous": sync["wall_s"] / per["wall_s"]},
         "vs_synchronous": sync["wall_s"] / grp["wall_s"],
         "roofline": roofline(grp["attempts"] * F, grp["wall_s"], window="timed region (persistent serving kernel)",
                              flops_per_attempt=F, attempts=int(grp["attempts"])),

@@ -81,30 +81,40 @@ class ServedVecEnv:
         """Whether the step with sequence numbers ``seq`` of ``env_ids`` has finished."""
         return bool(self.queue.done(env_ids, seq).all())
 
-    def collect(self, env_ids: np.ndarray, rng: Optional[np.random.Generator] = None
-                ) -> Tuple[np.ndarray, np.ndarray]:
+    def done_mask(self, env_ids: np.ndarray, seq: np.ndarray) -> np.ndarray:
+        """Per env of ``env_ids``: whether its step with sequence number ``seq`` has finished."""
+        return self.queue.done(env_ids, seq)
+
+    def collect(self, env_ids: np.ndarray, rng=None, record: bool = True) -> Tuple[np.ndarray, np.ndarray]:
         """Observations (len(env_ids), N) and failures of the finished step of ``env_ids``
-        (also written into ``observations`` / ``failures``); failed envs get new scenes
-        for their next step, drawn from ``rng`` (default: the env's own generator)."""
+        (also written into ``observations`` / ``failures`` unless ``record`` is False);
+        failed envs get new scenes for their next step, drawn from ``rng``: a generator,
+        a function of the env id returning one (one draw per env), or None (the env's
+        own generator)."""
         ids = np.asarray(env_ids)
         state, misc = self.queue.results(ids)
         obs = state[1].astype(np.float32)
         fail = misc[:, 2] != 1.0
-        # new arrays each time: observations returned by earlier steps stay as they were
-        prev_o = self.observations if self.observations is not None else \
-            np.zeros((self.num_envs, self.num_bubbles), dtype=np.float32)
-        prev_f = self.failures if self.failures is not None else np.zeros(self.num_envs, dtype=bool)
-        self.observations, self.failures = prev_o.copy(), prev_f.copy()
-        self.observations[ids] = obs
-        self.failures[ids] = fail
+        if record:
+            # new arrays each time: observations returned by earlier steps stay as they were
+            prev_o = self.observations if self.observations is not None else \
+                np.zeros((self.num_envs, self.num_bubbles), dtype=np.float32)
+            prev_f = self.failures if self.failures is not None else np.zeros(self.num_envs, dtype=bool)
+            self.observations, self.failures = prev_o.copy(), prev_f.copy()
+            self.observations[ids] = obs
+            self.failures[ids] = fail
         if self.scene is not None and fail.any():
             self.reset_envs(ids[fail], rng)
         return obs, fail
 
-    def reset_envs(self, env_ids, rng: Optional[np.random.Generator] = None) -> None:
-        """New positions for ``env_ids`` (radii kept), taken by their next step."""
+    def reset_envs(self, env_ids, rng=None) -> None:
+        """New positions for ``env_ids`` (radii kept), taken by their next step; ``rng`` as
+        for ``collect``."""
         env_ids = np.asarray(env_ids, dtype=np.int64)
-        X0 = self.scene(env_ids.size, self.rng if rng is None else rng)
+        if callable(rng):
+            X0 = np.concatenate([self.scene(1, rng(int(e))) for e in env_ids], axis=0)
+        else:
+            X0 = self.scene(env_ids.size, self.rng if rng is None else rng)
         state, up, cm = self.model.scene_rows(env_ids, None, X0)
         self.queue.stage(env_ids, state, up, cm)
         self._reset[env_ids] = True

@@ -321,8 +321,9 @@ def test_served_vec_env_equals_blocking_env_step():
 
 def test_overlapped_rollout_equals_synchronous():
     """Four rollout steps of 256 envs (N = 2) under a deterministic policy: the envs split
-    into three groups that step on their own see the observations, actions and failures
-    of one group and of the synchronous ServedVecEnv.step loop, bit for bit."""
+    into three groups that step on their own, and every env stepping on its own (policy
+    batches of >= 16 finished envs), see the observations, actions and failures of one
+    group and of the synchronous ServedVecEnv.step loop, bit for bit."""
     from coupledbubble_control.envs import ServedVecEnv
     from coupledbubble_control.rollout import OverlappedRollout
 
@@ -332,7 +333,7 @@ def test_overlapped_rollout_equals_synchronous():
 
     steps = 4
     got = {}
-    for groups in (0, 1, 3):   # 0: the synchronous loop
+    for groups in (0, 1, 3, "env"):   # 0: the synchronous loop
         print(f"\n[rollout] N=2 ns=256, {steps} steps, groups {groups}", flush=True)
         w, m, _ = _models(2, 256, seed=31)
         obs0 = m.runtime.host_buffers.state.actual_state[1, :, :2].astype(np.float32)
@@ -349,9 +350,9 @@ def test_overlapped_rollout_equals_synchronous():
                     fails.append(f)
                 got[groups] = {"observations": np.stack(obs), "actions": np.stack(acts), "failures": np.stack(fails)}
             else:
-                got[groups] = OverlappedRollout(env, policy, groups=groups).collect(obs0, steps)
+                got[groups] = OverlappedRollout(env, policy, groups=groups, min_batch=16).collect(obs0, steps)
         finally:
             env.close()
-    for groups in (1, 3):
+    for groups in (1, 3, "env"):
         for k in ("observations", "actions", "failures"):
             np.testing.assert_array_equal(got[groups][k], got[0][k], err_msg=f"groups {groups}: {k}")
