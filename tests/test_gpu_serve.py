@@ -66,12 +66,21 @@ def _rows(m, ids):
             db.state.actual_time.cpu().numpy()[ids], db.status.status_flags.cpu().numpy()[ids])
 
 
-@pytest.mark.parametrize("n,ns,variant", [(2, 256, None), (4, 128, None), (2, 128, "group")])
-def test_serve_equals_plain_bitwise(n, ns, variant):
+@pytest.mark.parametrize("n,ns,variant,exact", [(2, 256, None, False), (4, 128, None, False),
+                                               (2, 128, "group", False), (4, 128, None, True),
+                                               (8, 64, None, True)])
+def test_serve_equals_plain_bitwise(n, ns, variant, exact):
+    """A served step equals a plain launch of the same step bit for bit; also in the exact
+    event mode (N = 8: the Gauss--Jordan kernel's exact build on the serving path)."""
+    import dataclasses
+
     from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
 
-    print(f"\n[serve==plain] N={n} ns={ns} variant={variant}", flush=True)
+    print(f"\n[serve==plain] N={n} ns={ns} variant={variant} exact={exact}", flush=True)
     _, ms, mp = _models(n, ns, seed=11, variant=variant)
+    if exact:
+        for m in (ms, mp):
+            m._solver_spec = dataclasses.replace(m._solver_spec, event_mode="exact", etol=1e-12)
     rng = np.random.default_rng(5)
     with ServeQueue(ms, ServeOptions(capacity=1024, max_attempts=200000, idle_us=2000)) as q:
         steps = [np.arange(ns), rng.choice(ns, size=ns // 3, replace=False)]
