@@ -42,8 +42,11 @@ def copy_kernel(mib: int = 512, reps: int = 3) -> int:
 
 
 def attempt_latency(cases: Iterable[Tuple[int, str, int]] = LATENCY_CASES, periods: float = 1.0,
-                    reps: int = 3) -> List[Dict]:
-    """Critical-path microseconds per attempt for (bubbles, mapping, systems) cases."""
+                    reps: int = 3, event_mode: str = "bracket") -> List[Dict]:
+    """Critical-path microseconds per attempt for (bubbles, mapping, systems) cases, in the
+    given collision-event mode ("exact" with etol 1e-12)."""
+    import dataclasses
+
     import torch
 
     from . import workloads as W
@@ -58,6 +61,8 @@ def attempt_latency(cases: Iterable[Tuple[int, str, int]] = LATENCY_CASES, perio
             lam = W.wavelength(w.mat, 25.0)
             w.X0 = W.sample_positions(rng, ns, n, 0.0, 0.9, 3 * w.R0.max(1) * 1e-6 / lam)
         m = W.build_model(w, variant=var)
+        if event_mode != "bracket":
+            m._solver_spec = dataclasses.replace(m._solver_spec, event_mode=event_mode, etol=1e-12)
         m.runtime.host_buffers.state.time_end[:] = 0.05
         m.solve(max_steps=1 << 20, kernel_steps=1 << 20)   # loads the code object
         db = m.runtime.device_buffers
@@ -82,7 +87,8 @@ def attempt_latency(cases: Iterable[Tuple[int, str, int]] = LATENCY_CASES, perio
         att = db.status.total_steps.cpu().numpy()
         info = m.runtime.kernel.info()
         ms = min(times)
-        out.append({"n": n, "mapping": info.mapping, "systems": ns, "kernel_ms": round(ms, 3),
+        out.append({"n": n, "mapping": info.mapping, "event_mode": event_mode, "systems": ns,
+                    "kernel_ms": round(ms, 3), "events": int((db.status.status_flags == 2).sum().item()),
                     "max_attempts": int(att.max()), "mean_attempts": round(float(att.mean()), 1),
                     "us_per_attempt": round(ms * 1e3 / max(int(att.max()), 1), 3), "vgprs": info.num_regs,
                     "scratch_bytes": info.local_bytes, "lds_bytes": info.lds_bytes})
@@ -119,6 +125,8 @@ def main(argv: Sequence[str] = None) -> None:
     p.add_argument("--cases", default="", help="latency: comma list like 2lane,4group (default: all)")
     p.add_argument("--systems", default="",
                    help="latency: comma list of system counts run for every case (default: each case's own)")
+    p.add_argument("--event-mode", default="bracket", choices=["bracket", "exact"],
+                   help="latency: collision-event mode of the solver")
     a = p.parse_args(argv)
     if a.what == "copy":
         print(f"copy kernel: {copy_kernel(a.mib)} bytes read and written per launch", flush=True)
@@ -131,7 +139,7 @@ def main(argv: Sequence[str] = None) -> None:
             cases = [c for c in LATENCY_CASES if f"{c[0]}{c[1]}" in want]
         if a.systems:
             cases = [(n, var, int(ns)) for n, var, _ in cases for ns in a.systems.split(",")]
-        for r in attempt_latency(cases):
+        for r in attempt_latency(cases, event_mode=a.event_mode):
             print(json.dumps(r), flush=True)
 
 
