@@ -105,18 +105,18 @@ __device__ __forceinline__ bool group_any(bool v)
     return x != 0;
 }
 
-// argmax of (value, id) pairs; ties resolved to the lower id
+// argmax of (value, id) pairs; ties resolved to the lower id.  The maximum
+// first, then the lowest id among the lanes that hold it (every lane when the
+// maximum is NaN, i.e. every candidate is NaN).
 template <int G>
 __device__ __forceinline__ void group_argmax(double& v, int& id)
 {
-    group_steps<G>([&](auto c) {
-        constexpr int ctl = decltype(c)::value;
-        const double ov = dpp_double<ctl>(v);
-        const int oid = dpp_int<ctl>(id);
-        const bool take = (ov > v) || (ov == v && oid < id);
-        v = take ? ov : v;
-        id = take ? oid : id;
-    });
+    double m = v;
+    group_steps<G>([&](auto c) { m = fmax(m, dpp_double<decltype(c)::value>(m)); });
+    int cand = (v == m || m != m) ? id : 0x7fffffff;
+    group_steps<G>([&](auto c) { cand = min(cand, dpp_int<decltype(c)::value>(cand)); });
+    v = m;
+    id = cand;
 }
 
 // Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
