@@ -323,17 +323,17 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                 for (int j = k; j <= n; ++j) pv[j] = ar[j];
                 used_r = true;
                 col_r = k;
-                inv_r = rcp_fast(ar[k]);
             }
             if (own_t) {
 #pragma unroll
                 for (int j = k; j <= n; ++j) pv[j] = at[j];
                 used_t = true;
                 col_t = k;
-                inv_t = rcp_fast(at[k]);
             }
             __syncthreads();
-            const double inv = rcp_fast(pv[k]);
+            const double inv = rcp_fast(pv[k]);  // the owner's pivot: its own 1 / a_kk too
+            inv_r = own_r ? inv : inv_r;
+            inv_t = own_t ? inv : inv_t;
             const double fr = own_r ? 0.0 : ar[k] * inv;
             const double ft = own_t ? 0.0 : at[k] * inv;
 #pragma unroll
