@@ -16,7 +16,7 @@ import torch  # noqa: F401  (load torch's HIP runtime before libcbc)
 _PKG_ROOT = Path(__file__).resolve().parents[3]
 LIB_PATH = Path(os.environ.get("CBC_LIB", _PKG_ROOT / "lib" / "libcbc.so"))
 
-CBC_ABI_VERSION = 2
+CBC_ABI_VERSION = 3
 
 CBC_OK = 0
 CBC_ERR_INVALID = -1
@@ -86,7 +86,7 @@ class BuffersC(ctypes.Structure):
 
 class PeriodSpecC(ctypes.Structure):
     _fields_ = [("period", _f64), ("first_window", _i32), ("n_windows", _i32),
-                ("max_r", _vp), ("mean_x", _vp)]
+                ("max_r", _vp), ("mean_x", _vp), ("resume", _i32)]
 
 
 class PlanInfoC(ctypes.Structure):

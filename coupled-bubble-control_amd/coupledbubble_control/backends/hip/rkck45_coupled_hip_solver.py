@@ -121,10 +121,12 @@ def launch(kernel, d_buffers, kernel_steps: int, stream=None) -> None:
 
 
 def solve_periodic(kernel, d_buffers, period: float, first_window: int, max_r: torch.Tensor,
-                   mean_x: torch.Tensor, max_steps: int = 1_000_000, stream=None) -> None:
+                   mean_x: torch.Tensor, max_steps: int = 1_000_000, stream=None, resume: bool = False) -> None:
     """One launch integrating to time_end in windows of ``period`` and recording window
     statistics (asynchronous).  At most ``max_steps`` attempts per system: a system that
-    runs out stays ACTIVE with its open window saved, and the next call resumes it."""
+    runs out stays ACTIVE with its open window saved, and the next call (``resume=True``)
+    continues it; such a continuation leaves systems that stopped on a failure (negative
+    flag) untouched."""
     lib = _abi.lib()
     _check_shapes(d_buffers, kernel.model_spec, kernel.execution_spec)
     _check_trig_range(d_buffers, kernel.model_spec)
@@ -135,7 +137,7 @@ def solve_periodic(kernel, d_buffers, period: float, first_window: int, max_r: t
             raise ValueError(f"{name} must be contiguous float64 of shape {(nw, es.ns, es.tile)}")
     bufs = _buffers_c(d_buffers)
     ps = _abi.PeriodSpecC(float(period), int(first_window), nw, _abi.ptr(max_r) if nw else None,
-                          _abi.ptr(mean_x) if nw else None)
+                          _abi.ptr(mean_x) if nw else None, int(bool(resume)))
     _abi.check(lib.cbc_solve_periodic(kernel.handle, ctypes.byref(bufs), ctypes.byref(ps), int(max_steps),
                                       _abi.stream_handle(stream)))
 
@@ -145,6 +147,15 @@ def horizon_order(d_buffers) -> torch.Tensor:
     the device: the longest-processing-time-first order for the work queue."""
     rem = d_buffers.state.time_end - d_buffers.state.actual_time
     return torch.argsort(rem, descending=True, stable=True).to(torch.int32)
+
+
+def active_horizon_order(d_buffers) -> torch.Tensor:
+    """The systems still ACTIVE (flag 0), longest remaining horizon first (int32, device):
+    the queue of a later launch round of one solve, which must leave systems that
+    stopped -- on a failure included -- untouched.  May be empty."""
+    idx = torch.nonzero(d_buffers.status.status_flags == 0).flatten()
+    rem = (d_buffers.state.time_end - d_buffers.state.actual_time).index_select(0, idx)
+    return idx.index_select(0, torch.argsort(rem, descending=True, stable=True)).to(torch.int32)
 
 
 def _check_order(order: torch.Tensor, ns: int, validate: bool):

@@ -112,6 +112,8 @@ class MixedEnsemble:
         ups = [m._execution_spec.ups for m in self.models]
         for i, slot in stream_slots(ups, len(self._streams)):
             m, order, st = self.models[i], orders[i], self._streams[slot]
+            if order.numel() == 0:
+                continue
             t = self.tuning[m._execution_spec.ups]
             with torch.cuda.stream(st):
                 hs.launch_queued(m.runtime.kernel, m.runtime.device_buffers, order, waves=t.waves,
@@ -120,16 +122,21 @@ class MixedEnsemble:
             main.wait_stream(st)
 
     def solve(self, kernel_steps: int = 1 << 20, max_launches: int = 64) -> int:
-        """Launch until no system of any bin is ACTIVE; returns the launch rounds."""
+        """Launch until no system of any bin is ACTIVE; returns the launch rounds.  Rounds
+        after the first queue only the systems still ACTIVE, so a system that stopped on a
+        failure (STEP_SIZE_LIMIT, NaN) stays stopped, as in the reference host loop."""
         rounds = 0
+        orders = None
         for _ in range(max_launches):
-            self.launch(kernel_steps)
+            self.launch(kernel_steps, orders=orders)
             rounds += 1
             main = torch.cuda.current_stream()
             main.synchronize()   # joined every side stream in launch()
             left = sum(hs.active_count(m.runtime.kernel, main) for m in self.models)
             if left == 0:
                 break
+            orders = [hs.active_horizon_order(m.runtime.device_buffers) for m in self.models]
+            self._orders = orders
         return rounds
 
     def capture(self, kernel_steps: int = 1 << 20):

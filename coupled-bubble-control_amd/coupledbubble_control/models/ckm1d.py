@@ -250,8 +250,9 @@ class CKM1D(Model):
         dev = db.state.actual_state.device
         max_r = torch.zeros((n_windows, es.ns, es.tile), dtype=torch.float64, device=dev)
         mean_x = torch.zeros((n_windows, es.ns, es.tile), dtype=torch.float64, device=dev)
-        for _ in range(max_launches):
-            solve_periodic(self.runtime.kernel, db, period, first_window, max_r, mean_x, max_steps, stream)
+        for k in range(max_launches):
+            solve_periodic(self.runtime.kernel, db, period, first_window, max_r, mean_x, max_steps, stream,
+                           resume=k > 0)
             if not bool((db.status.status_flags == 0).any().item()):
                 break
         return max_r, mean_x

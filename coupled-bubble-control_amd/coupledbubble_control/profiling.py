@@ -110,9 +110,9 @@ def param_steps(steps: int = 2, periods_per_step: float = 10.0) -> int:
     db = m.runtime.device_buffers
     max_r = torch.zeros((w.n_windows, w.ns, 2), dtype=torch.float64, device="cuda")
     mean_x = torch.zeros_like(max_r)
-    for _ in range(steps):
+    for k in range(steps):
         db.state.time_end.add_(periods_per_step).clamp_(max=200.0)
-        solve_periodic(m.runtime.kernel, db, w.period, w.first_window, max_r, mean_x)
+        solve_periodic(m.runtime.kernel, db, w.period, w.first_window, max_r, mean_x, resume=k > 0)
     torch.cuda.synchronize()
     return int(db.status.total_steps.to(torch.int64).sum().item())
 

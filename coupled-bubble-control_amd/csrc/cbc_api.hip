@@ -240,8 +240,7 @@ int32_t cbc_plan_query(const cbc_plan* plan, cbc_plan_info* out)
     cbc::KernelAttr at{};
     const bool lane = plan->mapping == CBC_MAP_LANE;
     hipError_t e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                        : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
-                                          plan->solver.event_mode == CBC_EVENT_EXACT, &at);
+                        : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
     const int spb = lane ? plan->block : cbc::group_systems_per_block(plan->exec.ups);
     out->mapping = plan->mapping;
@@ -344,6 +343,7 @@ int32_t cbc_solve(cbc_plan* plan, const cbc_buffers* buf, int32_t kernel_steps, 
     int32_t launches = 0;
     for (int64_t l = 0; l < max_launches; ++l) {
         auto t0 = std::chrono::steady_clock::now();
+        a.resume = l > 0;
         hipError_t e = zero_async(plan->d_active, 1, st);
         if (e != hipSuccess) return hip_fail(e, "zero active counter");
         e = launch_kernel(plan, a, st);
@@ -381,6 +381,7 @@ int32_t cbc_solve_periodic(cbc_plan* plan, const cbc_buffers* buf, const cbc_per
     a.n_windows = period->n_windows;
     a.max_r = period->max_r;
     a.mean_x = period->mean_x;
+    a.resume = period->resume != 0;
     a.active_count = nullptr;
     hipError_t e = launch_kernel(plan, a, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch (periodic)");
@@ -406,8 +407,7 @@ static int32_t queued_args(cbc_plan* plan, const cbc_buffers* buf, const int32_t
         if (e != hipSuccess) return hip_fail(e, "device query");
         cbc::KernelAttr at{};
         e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
-                                   plan->solver.event_mode == CBC_EVENT_EXACT, &at);
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
         if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
         // waves resident per SIMD from the register budget (512 per lane on gfx950)
         const int regs = at.num_regs > 0 ? at.num_regs : 512;
@@ -496,6 +496,7 @@ int32_t cbc_solve_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t* 
     const int64_t max_launches = (max_steps + kernel_steps - 1) / kernel_steps;
     int32_t launches = 0;
     for (int64_t l = 0; l < max_launches; ++l) {
+        a.resume = l > 0;
         hipError_t e = queued_launch(plan, a, waves, st);
         if (e != hipSuccess) return hip_fail(e, "persistent kernel launch");
         e = hipMemcpyAsync(plan->h_active, plan->d_active, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
@@ -678,8 +679,7 @@ int32_t cbc_serve_create(cbc_plan* plan, const cbc_buffers* buf, const cbc_serve
         const bool lane = plan->mapping == CBC_MAP_LANE;
         cbc::KernelAttr at{};
         e = lane ? cbc::lane_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at)
-                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field,
-                                   plan->solver.event_mode == CBC_EVENT_EXACT, &at);
+                 : cbc::group_attr(plan->exec.ups, plan->model.nk, plan->model.field, &at);
         if (e != hipSuccess) {
             serve_free(s);
             return hip_fail(e, "hipFuncGetAttributes");

@@ -106,8 +106,11 @@ __device__ __forceinline__ bool group_any(bool v)
 }
 
 // argmax of (value, id) pairs; ties resolved to the lower id.  The maximum
-// first, then the lowest id among the lanes that hold it (every lane when the
-// maximum is NaN, i.e. every candidate is NaN).
+// first, then the lowest id among the lanes that hold it.  fmax ignores a NaN
+// operand, so NaN values lose to any number; the maximum is NaN only when every
+// value is NaN, and then every lane is a candidate.  (The Gauss--Jordan column
+// search marks used rows -1: when every unused row is NaN the maximum is -1 and
+// a used row is picked; group_rhs makes that solve's result NaN.)
 template <int G>
 __device__ __forceinline__ void group_argmax(double& v, int& id)
 {
@@ -300,7 +303,15 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             dy[2] = d2;
             dy[3] = d3;
         } else {
-        // Gauss--Jordan with partial pivoting across the group
+        // Gauss--Jordan with partial pivoting across the group.  Every solution
+        // slot starts as NaN (the barrier above ordered this after the previous
+        // solve's reads): a column that finds no finite unused row (NaN entries)
+        // picks a used row again, so some slot is never written and the
+        // accelerations come out NaN, never stale values of the previous solve.
+        if (vu) {
+            L.sol[i] = __builtin_nan("");
+            L.sol[N + i] = __builtin_nan("");
+        }
         bool used_r = !vu, used_t = !vu;
         int col_r = 0, col_t = 0;
         double inv_r = 0.0, inv_t = 0.0;

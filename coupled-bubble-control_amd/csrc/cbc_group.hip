@@ -4,11 +4,11 @@
 
 namespace cbc {
 
-const void* group_pick_n2(int nk, int field, bool exact);
-const void* group_pick_n3(int nk, int field, bool exact);
-const void* group_pick_n4(int nk, int field, bool exact);
-const void* group_pick_n8(int nk, int field, bool exact);
-const void* group_pick_n16(int nk, int field, bool exact);
+const void* group_pick_n2(int nk, int field);
+const void* group_pick_n3(int nk, int field);
+const void* group_pick_n4(int nk, int field);
+const void* group_pick_n8(int nk, int field);
+const void* group_pick_n16(int nk, int field);
 
 static int pow2_up(int v)
 {
@@ -17,25 +17,25 @@ static int pow2_up(int v)
     return t;
 }
 
-static const void* group_pick_any(int ups, int nk, int field, bool exact)
+static const void* group_pick_any(int ups, int nk, int field)
 {
     switch (ups) {
-    case 2: return group_pick_n2(nk, field, exact);
-    case 3: return group_pick_n3(nk, field, exact);
-    case 4: return group_pick_n4(nk, field, exact);
-    case 8: return group_pick_n8(nk, field, exact);
-    case 16: return group_pick_n16(nk, field, exact);
+    case 2: return group_pick_n2(nk, field);
+    case 3: return group_pick_n3(nk, field);
+    case 4: return group_pick_n4(nk, field);
+    case 8: return group_pick_n8(nk, field);
+    case 16: return group_pick_n16(nk, field);
     default: return nullptr;
     }
 }
 
-bool group_supported(int ups, int nk, int field) { return group_pick_any(ups, nk, field, false) != nullptr; }
+bool group_supported(int ups, int nk, int field) { return group_pick_any(ups, nk, field) != nullptr; }
 
 int group_systems_per_block(int ups) { return 64 / pow2_up(ups); }
 
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream)
 {
-    const void* fn = group_pick_any(a.ups, a.nk, a.field, a.event_exact != 0);
+    const void* fn = group_pick_any(a.ups, a.nk, a.field);
     if (!fn) return hipErrorInvalidValue;
     const int spb = group_systems_per_block(a.ups);
     const int grid = (a.ns + spb - 1) / spb;
@@ -45,15 +45,15 @@ hipError_t group_launch(const LaunchArgs& a, hipStream_t stream)
 
 hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream)
 {
-    const void* fn = group_pick_any(a.ups, a.nk, a.field, a.event_exact != 0);
+    const void* fn = group_pick_any(a.ups, a.nk, a.field);
     if (!fn || !(a.queue || a.ring) || !a.queue_head || waves <= 0) return hipErrorInvalidValue;
     void* args[] = {const_cast<LaunchArgs*>(&a)};
     return hipLaunchKernel(fn, dim3(waves), dim3(64), args, 0, stream);
 }
 
-hipError_t group_attr(int ups, int nk, int field, bool exact, KernelAttr* out)
+hipError_t group_attr(int ups, int nk, int field, KernelAttr* out)
 {
-    const void* fn = group_pick_any(ups, nk, field, exact);
+    const void* fn = group_pick_any(ups, nk, field);
     if (!fn) return hipErrorInvalidValue;
     hipFuncAttributes fa;
     hipError_t e = hipFuncGetAttributes(&fa, fn);

@@ -45,6 +45,15 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
     const size_t NST = NS * TILE;
     const size_t base = (size_t)s * TILE;
     R.s = s;
+    // a failure stop is final within one solve call: later launches leave it untouched
+    if (!it && a.resume) {
+        const int f = a.flags[s];
+        if (f < 0) {
+            R.flag = f;
+            R.vu = false;
+            return false;
+        }
+    }
     R.vu = i < N;
     const int iu = R.vu ? i : 0;
     R.t = it ? 0.0 : a.t[s];
@@ -167,7 +176,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
 // One attempt for every group of the wave (uniform: contains barriers).  `busy`
 // marks groups with a live system; returns true for a busy group whose system
 // stopped this attempt.
-template <int N, int NK, int FIELD, bool EXACT>
+template <int N, int NK, int FIELD>
 __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupLds<N>& L, GroupRun<N, NK>& R,
                                               bool busy)
 {
@@ -251,28 +260,23 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     }
     const double ratio = group_min<G>(ratio_u);
     const bool bad = group_any<G>(bad_u);
-    // exact event mode: the gap's root on the step, for groups that cross (the
-    // Gauss--Jordan kernels for N = 8, 16 come in a bracket-only and an exact build)
-    constexpr bool kExact = EXACT;
+    // exact event mode: the gap's root on the step, for groups that cross
     double theta = 1.0;
-    if constexpr (kExact) {
-        if (a.event_exact) {
-            const bool need = busy && R.g_old * g_new < 0.0 && !(fabs(g_new) < a.etol);
-            if (__any(need))
-                theta = group_event_root<N>(L, i, vu, R.P.rcol, y, ynew, h, a.col_factor, R.g_old, g_new);
-        }
+    if (a.event_exact) {
+        const bool need = busy && R.g_old * g_new < 0.0 && !(fabs(g_new) < a.etol);
+        if (__any(need))
+            theta = group_event_root<N>(L, i, vu, R.P.rcol, y, ynew, h, a.col_factor, R.g_old, g_new);
     }
     if (!busy) return false;
 
     const int s = R.s;
-    bool crossing = R.g_old * g_new < 0.0;
-    if constexpr (kExact) crossing = crossing && !(a.event_exact && fabs(g_new) < a.etol);
+    const bool crossing = R.g_old * g_new < 0.0 && !(a.event_exact && fabs(g_new) < a.etol);
     if (crossing) {
         if (h <= a.min_step) {
             R.flag = kStepSizeLimit;
             return true;
         }
-        if (kExact && a.event_exact) {
+        if (a.event_exact) {
             h = h * fmin(theta, kEventThetaMax);
         } else {
             const double aold = fabs(R.g_old);
@@ -378,7 +382,7 @@ __device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const G
     }
 }
 
-template <int N, int NK, int FIELD, bool EXACT>
+template <int N, int NK, int FIELD>
 __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
 {
     using Sh = GroupShape<N>;
@@ -486,7 +490,7 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
         }
         idle_since = 0;
         ++since_poll;
-        const bool stop = group_attempt<N, NK, FIELD, EXACT>(a, i, L, R, busy);
+        const bool stop = group_attempt<N, NK, FIELD>(a, i, L, R, busy);
         if (busy && (stop || ++used >= a.kernel_steps)) {
             group_finish<N, NK>(a, i, R);
             busy = false;
@@ -495,18 +499,11 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
     }
 }
 
-// The kernels with the redundant solve (N <= 4) carry the exact event block in
-// every build and test the mode at run time.  Compiled into the Gauss--Jordan
-// kernels (N = 8, 16) that block made bracket-mode runs return NaN, so those
-// bubble counts have a bracket-only build, used for the bracket mode, and an
-// exact build, used only for the exact mode.
+// Every build carries the exact event block and tests the mode at run time.
 template <int N>
-const void* group_pick(int nk, int field, bool exact)
+const void* group_pick(int nk, int field)
 {
-    constexpr bool kAlways = GroupShape<N>::redundant;
-#define CBC_G(NK_, F_)                                                                   \
-    return exact || kAlways ? reinterpret_cast<const void*>(&group_kernel<N, NK_, F_, true>) \
-                            : reinterpret_cast<const void*>(&group_kernel<N, NK_, F_, kAlways>)
+#define CBC_G(NK_, F_) return reinterpret_cast<const void*>(&group_kernel<N, NK_, F_>)
     if (nk == 1) {
         if (field == kFieldConst) CBC_G(1, kFieldConst);
         if (field == kFieldSwN) CBC_G(1, kFieldSwN);

@@ -2,5 +2,5 @@
 #include "cbc_group_kernel.h"
 
 namespace cbc {
-const void* group_pick_n8(int nk, int field, bool exact) { return group_pick<8>(nk, field, exact); }
+const void* group_pick_n8(int nk, int field) { return group_pick<8>(nk, field); }
 }  // namespace cbc

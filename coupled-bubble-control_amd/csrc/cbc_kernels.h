@@ -15,6 +15,8 @@ struct LaunchArgs {
     double atol, rtol, min_step, max_step, etol;
     double col_factor;         // 1 + collision threshold
     int event_exact;           // CBC_EVENT_EXACT
+    int resume;                // a later launch of one solve call: systems whose flag is
+                               // negative (a failure stop) stay stopped, untouched
 
     double* t;
     const double* t_end;
@@ -109,7 +111,7 @@ bool lane_supported(int ups, int nk, int field);
 // Group mapping: next_pow2(ups) lanes per system, one wave per workgroup.
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
 hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream);
-hipError_t group_attr(int ups, int nk, int field, bool exact, KernelAttr* out);
+hipError_t group_attr(int ups, int nk, int field, KernelAttr* out);
 bool group_supported(int ups, int nk, int field);
 int group_systems_per_block(int ups);
 

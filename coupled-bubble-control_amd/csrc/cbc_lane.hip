@@ -223,6 +223,15 @@ __device__ __forceinline__ bool lane_begin(const LaunchArgs& a, int s, LaneRun<N
     const size_t NST = NS * TILE;
     const size_t base = (size_t)s * TILE;
     R.s = s;
+    // a failure stop is final within one solve call (the host loop of the
+    // oracle stops such a system for good): later launches leave it untouched
+    if (!it && a.resume) {
+        const int f = a.flags[s];
+        if (f < 0) {
+            R.flag = f;
+            return false;
+        }
+    }
     R.t = it ? 0.0 : a.t[s];
     R.t_end = it ? it->t_end : a.t_end[s];
     R.h = (it && it->h0 > 0.0) ? it->h0 : a.h[s];

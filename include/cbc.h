@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CBC_ABI_VERSION 2
+#define CBC_ABI_VERSION 3
 
 /* return codes */
 #define CBC_OK 0
@@ -132,6 +132,9 @@ typedef struct cbc_period_spec {
     int32_t n_windows;
     double *max_r;
     double *mean_x;
+    int32_t resume;       /* nonzero: a continuation launch of the same integration (a
+                             budget-limited call repeated); systems whose flag is negative
+                             (a failure stop) are left untouched */
 } cbc_period_spec;
 
 typedef struct cbc_plan_info {

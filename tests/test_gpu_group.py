@@ -110,3 +110,34 @@ def test_auto_mapping_for_two_bubbles(device):
         m = workloads.build_model(w)
         m.solve(max_steps=1 << 20, kernel_steps=1 << 20)
         assert m.runtime.kernel.info().mapping == want, (ns, cus)
+
+
+def test_no_solver_kernel_uses_scratch(device):
+    """Every kernel the AUTO mapping selects (lane: N = 1, 2; group: N = 2, 3, 4, 8, 16;
+    one or two harmonics; the three field types) keeps its state in registers and LDS: no
+    private (scratch) bytes per lane, with the exact-event block compiled into every
+    kernel.  The lane kernels for N = 3, 4 (chosen only on request) spill, as recorded."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+
+    print("case: scratch bytes of every solver kernel", flush=True)
+    seen = []
+    for variant, ns_list in ((KernelVariant.LANE, (1, 2, 3, 4)), (KernelVariant.GROUP, (2, 3, 4, 8, 16))):
+        for n in ns_list:
+            for nk in (1, 2):
+                for field in ("CONST", "SW_N", "SW_A"):
+                    ns = 4
+                    w = workloads.Workload(f"scratch_N{n}", n, np.full((ns, n), 30.0),
+                                           np.tile(np.arange(n) / (2.0 * n), (ns, 1)),
+                                           np.full((nk, ns), 0.5), np.full((nk, ns), 25.0), np.zeros((nk, ns)),
+                                           np.full(ns, 0.1), field=field)
+                    m = workloads.build_model(w, variant=variant)
+                    m._ensure_buffers()
+                    m.runtime.solver = m._make_solver(m.runtime.backend, m.runtime.variant)
+                    info = m.runtime.kernel.info()
+                    assert info.mapping == variant.value
+                    seen.append((variant.value, n, nk, field, info.num_regs, info.local_bytes))
+                    if variant == KernelVariant.GROUP or n <= 2:
+                        assert info.local_bytes == 0, seen[-1]
+    print(f"  {len(seen)} kernels, registers {min(s[4] for s in seen)}..{max(s[4] for s in seen)}; "
+          f"spilling: {[s for s in seen if s[5]]}", flush=True)

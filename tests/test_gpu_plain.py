@@ -49,13 +49,36 @@ def _assert_states(m, ref, n, rtol=RTOL):
     return err.max()
 
 
-def _assert_counts(m, cnt, frac=0.02):
-    got = m.runtime.host_buffers.status.total_steps.astype(np.int64)
-    want = cnt["total_steps"]
-    rel = np.abs(got - want) / np.maximum(want, 1)
-    assert rel.max() <= frac, f"step-count mismatch {rel.max():.3%} (got {got[np.argmax(rel)]}, want {want[np.argmax(rel)]})"
-    total_rel = abs(int(got.sum()) - int(want.sum())) / max(int(want.sum()), 1)
-    assert total_rel <= frac / 4
+# Rejected attempts are 2-6 % of a system's attempts; a rounding-level difference in an
+# error ratio near 1 moves single rejections, so a system's count is held to REJ_FRAC or
+# REJ_SLACK (REJ_SLACK_FAIL for a system that stops at a singularity, whose last attempts
+# are a cascade of rejections that each at least halve h) and the ensemble's total to frac.
+REJ_FRAC = 0.05
+REJ_SLACK = 5
+REJ_SLACK_FAIL = 10
+
+
+def _assert_counts(m, cnt, frac=0.02, got_total=None, got_rejected=None, failed=None):
+    """Accepted and rejected step counts, each against the oracle's: accepted within
+    `frac` per system (the bar on step counts) and frac / 4 in total; rejected as above."""
+    hb = m.runtime.host_buffers.status
+    tot = (hb.total_steps if got_total is None else got_total).astype(np.int64)
+    rej = (hb.rejected_steps if got_rejected is None else got_rejected).astype(np.int64)
+    acc = tot - rej
+    want_rej = cnt["rejected_steps"].astype(np.int64)
+    want_acc = cnt["total_steps"].astype(np.int64) - want_rej
+    rel = np.abs(acc - want_acc) / np.maximum(want_acc, 1)
+    k = int(np.argmax(rel))
+    assert rel.max() <= frac, f"accepted-step mismatch {rel.max():.3%} (system {k}: got {acc[k]}, want {want_acc[k]})"
+    assert abs(int(acc.sum()) - int(want_acc.sum())) <= frac / 4 * max(int(want_acc.sum()), 1)
+    dr = np.abs(rej - want_rej)
+    slack = REJ_SLACK if failed is None else np.where(failed, REJ_SLACK_FAIL, REJ_SLACK)
+    bad = dr > np.maximum(REJ_FRAC * want_rej, slack)
+    assert not bad.any(), (f"rejected-step mismatch at {np.flatnonzero(bad)[:8]}: got {rej[bad][:8]}, "
+                           f"want {want_rej[bad][:8]}")
+    assert abs(int(rej.sum()) - int(want_rej.sum())) <= max(frac * int(want_rej.sum()), REJ_SLACK)
+    print(f"  counts: accepted max rel dev {rel.max():.2%}, rejected max dev {int(dr.max())} "
+          f"(total {int(rej.sum())} vs {int(want_rej.sum())})", flush=True)
     return rel.max()
 
 
@@ -297,3 +320,103 @@ def test_period_windows_resume_across_launches(device):
         np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
     np.testing.assert_array_equal(m1.runtime.device_buffers.state.actual_state.cpu().numpy(),
                                   m2.runtime.device_buffers.state.actual_state.cpu().numpy())
+
+
+def _failing_mix(seed=11, ns=256, t_end=3.0):
+    """Strongly driven small bubbles (R0 log-uniform in [5, 50] um, PA in [0.5, 2] bar):
+    about a third of the systems stop with STEP_SIZE_LIMIT near tau = 1.1-1.2 after ~2 000
+    attempts (a wall speed reaching the sound speed), the others run to tau = 3 with up to
+    ~8 700 attempts."""
+    from coupledbubble_control import workloads
+
+    w = workloads.plain_cpu(seed=seed, ns=ns)
+    rng = np.random.default_rng(5)
+    w.R0 = np.exp(rng.uniform(np.log(5.0), np.log(50.0), size=(ns, 2)))   # smaller than the gap's R0
+    w.PA = rng.uniform(0.5, 2.0, size=(2, ns))
+    w.t_end[:] = t_end
+    return w
+
+
+@pytest.mark.parametrize("variant", ["lane", "group"])
+def test_failure_stops_stay_stopped_across_launches(device, variant):
+    """With 300 attempts per launch the solve takes ~30 launches; systems that stop on a
+    failure in an early launch must stay stopped while the others run on (the oracle's
+    host loop stops such a system for good).  Against one launch of the same kernel:
+    every system bit for bit (a failed system restarted by a later launch would gain an
+    attempt per launch).  Against the oracle: flags, accepted and rejected counts, states
+    and dense samples."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+
+    print(f"case: failure stops across launches, 256xN2 {variant}, kernel_steps 300, ndo 16", flush=True)
+    w = _failing_mix()
+    one = workloads.build_model(w, variant=KernelVariant(variant))
+    one.solve(max_steps=10 ** 8, kernel_steps=10 ** 6, sync_to_host=True)
+    many = workloads.build_model(w, variant=KernelVariant(variant))
+    many.solve(max_steps=10 ** 8, kernel_steps=300, sync_to_host=True)
+    a, b = one.runtime.host_buffers, many.runtime.host_buffers
+    for name, x, y in (("flags", a.status.status_flags, b.status.status_flags),
+                       ("attempts", a.status.total_steps, b.status.total_steps),
+                       ("rejected", a.status.rejected_steps, b.status.rejected_steps),
+                       ("state", a.state.actual_state, b.state.actual_state),
+                       ("t", a.state.actual_time, b.state.actual_time),
+                       ("h", a.state.time_step, b.state.time_step)):
+        np.testing.assert_array_equal(x, y, err_msg=name)
+
+    m, ref, cnt = _run_pair(w, kernel_steps=300, ndo=16, variant=KernelVariant(variant))
+    hb = m.runtime.host_buffers
+    failed = ref["status_flags"] < 0
+    assert 40 <= failed.sum() <= 160, failed.sum()
+    assert cnt["total_steps"].max() > 20 * 300          # the long runs need 20+ launches
+    np.testing.assert_array_equal(hb.status.status_flags, ref["status_flags"])
+    _assert_counts(m, cnt, failed=failed)
+    _assert_states(m, ref, 2)
+    # every launch records a dense point at its start (the reference's load), so a system
+    # whose attempt count differs from the oracle's by rounding may place one point
+    # differently; the others match exactly
+    same = hb.status.total_steps.astype(np.int64) == cnt["total_steps"]
+    ddi = np.abs(hb.dense.dense_index.astype(np.int64) - ref["dense_index"])
+    assert np.all(ddi[same] == 0) and np.all(ddi <= 1), (np.flatnonzero(ddi), same.sum())
+    # dense times of the failed systems and their stop times: dense samples sit at the first
+    # accepted step past each sample time, and on these strongly driven runs the step
+    # placements of the two implementations drift apart by a fraction of a step (3e-5
+    # relative seen) while the attempt counts stay equal
+    for s in np.flatnonzero(failed & same):
+        k = hb.dense.dense_index[s]
+        np.testing.assert_allclose(hb.dense.dense_time[:k, s], ref["dense_time"][:k, s], rtol=1e-4)
+    np.testing.assert_allclose(hb.state.actual_time[failed], ref["actual_time"][failed], rtol=1e-4)
+
+
+def test_failure_stops_queued_and_ensemble_equal_plain(device):
+    """The same mix through the persistent queue (cbc_solve_queued, budget 300 per launch)
+    and the mixed ensemble (launch rounds of 300 attempts): bit for bit the plain solve."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends.hip import rkck45_coupled_hip_solver as hs
+    from coupledbubble_control.ensemble import MixedEnsemble
+
+    print("case: failure stops, queued and ensemble vs plain, 256xN2, budget 300", flush=True)
+    w = _failing_mix(ns=192)
+
+    def snap(mm):
+        d = mm.runtime.device_buffers
+        return {k: v.detach().cpu().numpy().copy() for k, v in (
+            ("state", d.state.actual_state), ("t", d.state.actual_time), ("h", d.state.time_step),
+            ("flags", d.status.status_flags), ("steps", d.status.total_steps),
+            ("rejected", d.status.rejected_steps))}
+
+    a = workloads.build_model(w)
+    a.solve(max_steps=10 ** 7, kernel_steps=300)
+    want = snap(a)
+    assert (want["flags"] < 0).sum() > 20
+    b = workloads.build_model(w)
+    b._ensure_buffers()
+    b.runtime.solver = b._make_solver(b.runtime.backend, b.runtime.variant)
+    launches = hs.solve_queued(b.runtime.kernel, b.runtime.device_buffers, kernel_steps=300, max_steps=10 ** 7)
+    assert launches > 20
+    c = workloads.build_model(w)
+    rounds = MixedEnsemble([c]).solve(kernel_steps=300, max_launches=64)
+    assert rounds > 20
+    for name, mm in (("queued", b), ("ensemble", c)):
+        got = snap(mm)
+        for k in want:
+            np.testing.assert_array_equal(got[k], want[k], err_msg=f"{name}: {k}")
