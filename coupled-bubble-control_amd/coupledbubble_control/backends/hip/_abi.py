@@ -107,6 +107,7 @@ class ServeViewC(ctypes.Structure):
 CBC_ENV_MAX_ACTIONS = 16
 CBC_ENV_MAX_PARTS = 8
 CBC_ENV_MAX_BUBBLES = 16
+CBC_ENV_STATIC_MAX_ENVS = 1 << 20   # envs served by the static workspace of cbc_env_post_static
 CBC_ENV_PART_STATE = 0
 CBC_ENV_PART_TARGET = 1
 CBC_ENV_PART_FIXED = 2
@@ -194,6 +195,8 @@ SIGNATURES = {
     "cbc_env_set_action": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _vp]),
     "cbc_env_post": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _i64, _vp,
                             ctypes.POINTER(EnvCountsC)]),
+    "cbc_env_post_static": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), _i64, _vp,
+                                   ctypes.POINTER(EnvCountsC)]),
     "cbc_env_reset": (_i32, [ctypes.POINTER(EnvSpecC), ctypes.POINTER(EnvBuffersC), ctypes.POINTER(EnvSceneC),
                              _vp, _i32, _i64, _vp, _vp, _vp, _vp]),
     "cbc_env_scene_positions": (_i32, [ctypes.POINTER(EnvSceneC), _i32, _i32, _vp, _vp, _vp]),

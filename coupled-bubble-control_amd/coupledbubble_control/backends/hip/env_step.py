@@ -150,9 +150,11 @@ class EnvKernels:
         _abi.check(_abi.lib().cbc_env_set_action(ctypes.byref(self.spec), ctypes.byref(b),
                                                  _abi.stream_handle(stream)))
 
-    def post(self, observed: int, stream=None) -> _abi.EnvCountsC:
+    def post(self, observed: int, stream=None, static_workspace: bool = False) -> _abi.EnvCountsC:
         """observe .. advance_time (:756-827) into fresh output tensors; returns the
-        counts (synchronises the stream, as the reference's step does)."""
+        counts (synchronises the stream, as the reference's step does).  With
+        ``static_workspace`` the call goes through cbc_env_post_static, which uses the
+        library's own device workspace instead of this object's (same results)."""
         dev = self.device
         out = StepOutputs(
             observations=torch.empty((self.ne, self.obs_dim), dtype=torch.float32, device=dev),
@@ -164,8 +166,9 @@ class EnvKernels:
         self._check_solver_buffers()
         b = self._buffers(out)
         counts = _abi.EnvCountsC()
-        _abi.check(_abi.lib().cbc_env_post(ctypes.byref(self.spec), ctypes.byref(b), int(observed),
-                                           _abi.stream_handle(stream), ctypes.byref(counts)))
+        fn = _abi.lib().cbc_env_post_static if static_workspace else _abi.lib().cbc_env_post
+        _abi.check(fn(ctypes.byref(self.spec), ctypes.byref(b), int(observed), _abi.stream_handle(stream),
+                      ctypes.byref(counts)))
         self.outputs = out
         self.observations = out.observations
         return counts

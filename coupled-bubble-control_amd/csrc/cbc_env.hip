@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "../../include/cbc.h"
 
@@ -444,7 +445,7 @@ int32_t check_spec(const cbc_env_spec* s)
     return CBC_OK;
 }
 
-int32_t check_buffers(const cbc_env_buffers* b, bool post)
+int32_t check_buffers(const cbc_env_buffers* b, bool post, bool need_workspace = true)
 {
     if (!b) return fail(CBC_ERR_INVALID, "env: null buffers");
     if (!b->actual_state || !b->history || !b->observations || !b->obs_bias || !b->obs_rscale ||
@@ -452,7 +453,7 @@ int32_t check_buffers(const cbc_env_buffers* b, bool post)
         return fail(CBC_ERR_INVALID, "env: null state, history, observation or counter buffer");
     if (post && (!b->actual_time || !b->time_step || !b->actual_event || !b->status_flags || !b->actions ||
                  !b->rewards || !b->failure || !b->success || !b->terminated || !b->time_outs || !b->done_ids ||
-                 !b->workspace))
+                 (need_workspace && !b->workspace)))
         return fail(CBC_ERR_INVALID, "env: null buffer for the post-step kernel");
     return CBC_OK;
 }
@@ -465,6 +466,71 @@ int32_t check_side(const cbc_scene_side& sd, int n, int N, const char* name)
         return fail(CBC_ERR_INVALID, "env: %s random scenes need u_gaps", name);
     if (n > 0 && sd.method != CBC_SCENE_FIXED && sd.random_align && !sd.u_align)
         return fail(CBC_ERR_INVALID, "env: %s random alignment needs u_align", name);
+    return CBC_OK;
+}
+
+// ---- the workspace of cbc_env_post_static -------------------------------------
+// A file-scope device array sized for CBC_ENV_STATIC_MAX_ENVS envs (4.1 MiB), so the
+// call needs no workspace argument and allocates nothing.  In host code `scratch`
+// names a host-side stand-in, not device memory: its device address on each GPU comes
+// from hipGetSymbolAddress, once per device, and only that address is ever used.
+constexpr int kStaticBlocks = (CBC_ENV_STATIC_MAX_ENVS + kBlock - 1) / kBlock;
+constexpr int64_t kStaticWords = (int64_t)kWsBlocks + 5 * (int64_t)kStaticBlocks + (int64_t)kStaticBlocks * kBlock;
+static_assert(CBC_ENV_STATIC_MAX_ENVS <= kMaxEnvs, "the static workspace serves at most the kernels' env limit");
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call there).
+int32_t static_workspace(uint32_t** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(CBC_ERR_UNSUPPORTED, "env_post_static: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(env scratch)");
+        if (!p) return fail(CBC_ERR_HIP, "hipGetSymbolAddress(env scratch) returned null");
+        g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CBC_OK;
+}
+
+__global__ void zero_ticket(uint32_t* ws) { ws[0] = 0u; }
+
+// cbc_env_post's launches and counts read-back on workspace `ws`.
+int32_t env_post_run(const cbc_env_spec* spec, cbc_env_buffers b, uint32_t* ws, int64_t observed, hipStream_t st,
+                     cbc_env_counts* counts)
+{
+    b.workspace = ws;
+    PostArgs a{*spec, b, observed};
+    void* args[] = {&a};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_post), dim3(n_blocks(spec->num_envs)),
+                                   dim3(kBlock), args, 0, st);
+    if (e != hipSuccess) return hip_fail(e, "env_post launch");
+    {
+        const uint32_t* cws = ws;
+        int nb = n_blocks(spec->num_envs);
+        int64_t* ids = b.done_ids;
+        void* cargs[] = {&cws, &nb, &ids};
+        e = hipLaunchKernel(reinterpret_cast<const void*>(&env_compact), dim3(nb), dim3(kBlock), cargs, 0, st);
+        if (e != hipSuccess) return hip_fail(e, "env_compact launch");
+    }
+    if (!counts) return CBC_OK;
+    uint32_t h[4];
+    e = hipMemcpyAsync(h, ws + kWsCounts, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "env_post counts readback");
+    counts->done = (int32_t)h[0];
+    counts->failures = (int32_t)h[1];
+    counts->successes = (int32_t)h[2];
+    counts->time_outs = (int32_t)h[3];
     return CBC_OK;
 }
 
@@ -503,30 +569,32 @@ int32_t cbc_env_post(const cbc_env_spec* spec, const cbc_env_buffers* buf, int64
     if (rc != CBC_OK) return rc;
     if (observed < 0) return fail(CBC_ERR_INVALID, "env_post: observed must be >= 0");
     if (spec->num_pa > 0 && !buf->actions) return fail(CBC_ERR_INVALID, "env_post: null actions");
-    PostArgs a{*spec, *buf, observed};
-    void* args[] = {&a};
+    return env_post_run(spec, *buf, buf->workspace, observed, (hipStream_t)stream, counts);
+}
+
+int32_t cbc_env_post_static(const cbc_env_spec* spec, const cbc_env_buffers* buf, int64_t observed, void* stream,
+                            cbc_env_counts* counts)
+{
+    int32_t rc = check_spec(spec);
+    if (rc != CBC_OK) return rc;
+    if (spec->num_envs > CBC_ENV_STATIC_MAX_ENVS)
+        return fail(CBC_ERR_INVALID, "env_post_static: %d envs exceed the static workspace's %d", spec->num_envs,
+                    CBC_ENV_STATIC_MAX_ENVS);
+    rc = check_buffers(buf, true, false);
+    if (rc != CBC_OK) return rc;
+    if (observed < 0) return fail(CBC_ERR_INVALID, "env_post_static: observed must be >= 0");
+    if (spec->num_pa > 0 && !buf->actions) return fail(CBC_ERR_INVALID, "env_post_static: null actions");
+    uint32_t* ws = nullptr;
+    rc = static_workspace(&ws);
+    if (rc != CBC_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_post), dim3(n_blocks(spec->num_envs)),
-                                   dim3(kBlock), args, 0, st);
-    if (e != hipSuccess) return hip_fail(e, "env_post launch");
-    {
-        const uint32_t* ws = buf->workspace;
-        int nb = n_blocks(spec->num_envs);
-        int64_t* ids = buf->done_ids;
-        void* cargs[] = {&ws, &nb, &ids};
-        e = hipLaunchKernel(reinterpret_cast<const void*>(&env_compact), dim3(nb), dim3(kBlock), cargs, 0, st);
-        if (e != hipSuccess) return hip_fail(e, "env_compact launch");
-    }
-    if (!counts) return CBC_OK;
-    uint32_t h[4];
-    e = hipMemcpyAsync(h, buf->workspace + kWsCounts, sizeof(h), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "env_post counts readback");
-    counts->done = (int32_t)h[0];
-    counts->failures = (int32_t)h[1];
-    counts->successes = (int32_t)h[2];
-    counts->time_outs = (int32_t)h[3];
-    return CBC_OK;
+    // the done-list ticket must start at zero (env_post's last workgroup leaves it zero,
+    // but a call on this array may have been cut short); a kernel, not a memset, so the
+    // call stays valid under stream capture
+    void* zargs[] = {&ws};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&zero_ticket), dim3(1), dim3(1), zargs, 0, st);
+    if (e != hipSuccess) return hip_fail(e, "env_post_static ticket reset");
+    return env_post_run(spec, *buf, ws, observed, st, counts);
 }
 
 int32_t cbc_env_reset(const cbc_env_spec* spec, const cbc_env_buffers* buf, const cbc_env_scene* scene,

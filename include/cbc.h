@@ -441,6 +441,19 @@ int32_t cbc_env_set_action(const cbc_env_spec *spec, const cbc_env_buffers *buf,
 int32_t cbc_env_post(const cbc_env_spec *spec, const cbc_env_buffers *buf, int64_t observed, void *stream,
                      cbc_env_counts *counts);
 
+/* No counterpart in the reference: cbc_env_post without the workspace argument, for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels, arguments and results
+ * as cbc_env_post; buf->workspace is ignored (may be NULL).  The workspace is a
+ * file-scope device array of the library sized for CBC_ENV_STATIC_MAX_ENVS envs
+ * (4.1 MiB), which every GPU that runs this call holds until the process ends; the call
+ * itself allocates nothing.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams, other threads) race and must not be issued.
+ * num_envs above CBC_ENV_STATIC_MAX_ENVS returns CBC_ERR_INVALID. */
+#define CBC_ENV_STATIC_MAX_ENVS (1 << 20)
+int32_t cbc_env_post_static(const cbc_env_spec *spec, const cbc_env_buffers *buf, int64_t observed, void *stream,
+                            cbc_env_counts *counts);
+
 /* final_termination's per-done-env outputs and reset_envs (:583-614) for the n
  * envs ids[0..n): final_observation (n, obs_dim), episode_return (n,),
  * episode_length (n,) (each may be NULL) are taken before the reset; then new
