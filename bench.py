@@ -8,8 +8,12 @@ bubbles with implicit radial/translational coupling):
   Requests of 64..1024 envs (one vector env's step: new PA1, PA2 ~ U[0, 1.5] bar,
   integrate K = 5 periods from the resident state and step size) arrive Poisson
   at 200 000 env-steps/s; failed envs and 0.1 % of envs per step (10 % per 100
-  steps) restart with fresh radii and positions.  A step is one request;
-  ``--steps`` requests are timed.  The same requests are then served by the
+  steps) restart with fresh radii and positions.  A step is one round of the
+  offered load: one request per vector env on average (33 requests, 16 384 env
+  steps, ~82 ms of arrivals).  The request stream holds ``--warmup`` rounds,
+  then the ``--steps`` timed rounds, then 3 more rounds that keep the load on
+  while the timed requests finish; value and latency cover the timed requests
+  only (steady state).  The same requests are then served by the
   plain path (one kernel launch per batch, per bubble count), and both paths
   are also driven closed-loop (each vector env resubmits as soon as its step
   completes) for ``--saturation-s`` seconds to measure capacity.  "Plain" is
@@ -17,8 +21,9 @@ bubbles with implicit radial/translational coupling):
   one batch (one launch per bubble count), and the next batch starts when all
   of its launches have ended; ``plain_per_bin`` also reports one such batch
   loop per bubble count.
-  value = integrated periods / wall seconds of the continuous path under the
-  Poisson load; latency = request arrival to completion.
+  value = integrated periods of the timed requests / the span from their first
+  arrival to their last completion, continuous path under the Poisson load;
+  latency = request arrival to completion, p50/p99 over the timed requests.
 * ``param_study`` -- 65 536 systems of N=2 on a 256 x 256 (PA1, PA2) grid over
   [0, 2] bar, R0 = (44, 60) um, 200 periods with per-period max R/R0 and mean x
   for periods 51..200, rtol = atol = 1e-10; a step is ``--periods-per-step``
@@ -38,16 +43,14 @@ bubbles with implicit radial/translational coupling):
   scenes), random actions; a step is one env.step (set_action, solve, observe,
   termination, rewards, advance_time, resets of finished episodes).  Reports
   step latency and the split between the solver and the rest of the step.
-* ``rollout`` -- PPO-style rollout collection over the serving queue: 16 384
-  resident envs of 2 bubbles (``--env-count``), 5 periods per step, failed envs
-  restarted with new positions, actions from an actor MLP (256x256 tanh, Beta
-  mean) on each env's observed positions.  The synchronous loop (every env steps,
-  then the policy runs on every env, as rl/ppo.py:249-300 does) against
-  ``--rollout-groups`` groups of envs stepping on their own with the policy run
-  on each finished group (value: the grouped loop's nominal periods/s, env steps
-  x 5 periods), and every env stepping on its own, the policy run on batches of
-  >= 1/16 of the envs whose steps finished (``per_env``; rollout.py); a step =
-  one rollout step of every env.
+* ``rollout`` -- PPO-style rollout collection over PosNBC1D's full step (rewards,
+  terminations, time-outs, episode resets): 16 384 envs of 2 bubbles
+  (``--env-count``, ``--env-bubbles``), 5 periods per step, actions from an actor
+  MLP (256x256 tanh, Beta mean) on the observations.  The synchronous loop (the
+  policy on every env, then PosNBC1D.step on every env, as rl/ppo.py:249-300
+  does; value: its periods/s) beside ServedPosNBC1D with ``--rollout-groups``
+  blocks of envs stepping on their own over the serving kernel and with one
+  block; a step = one rollout step of every env.
 
 Every line carries the FP64 roofline of the dominant kernel (algorithmic FLOPs
 per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
@@ -76,14 +79,12 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
 # statistics (profiles/lane_kernel_param_study_65536xN2.md).  Only that launch length was
 # measured; other --periods-per-step report no traffic.
 PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
-# Fabric bytes of the two persistent serving kernels over the timed region of the default
-# serving run (1 100 requests at 2e5 env-steps/s), the same counters and factors: 4.33 GB read +
-# 0.36 GB written (7 914 + 649 B per env step).  The writes scale with the env steps; the reads
-# are mostly the waves' polls of the request ring and scale with the time the region lasts (7.7 s
-# under counter collection, ~5.6 s without).  Ring, control words and outputs live in mapped
-# host memory, so this bounds the HBM bytes from above.  Other request counts or rates report
-# no traffic.
-SERVING_TRAFFIC_BYTES_1100 = 4682015437
+# Fabric bytes per env step served by the two persistent serving kernels, measured with
+# rocprofv3 over a default serving run (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of
+# a copy kernel of known size), total bytes of all their dispatches / env steps served;
+# ring, control words and outputs live in mapped host memory, so this bounds the HBM bytes
+# from above.  None until measured for the current kernels.
+SERVING_TRAFFIC_BYTES_PER_ENV_STEP = None
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
@@ -106,18 +107,24 @@ def parse():
     p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env", "rollout"],
                    default="serving")
     p.add_argument("--steps", type=int, default=None,
-                   help="serving: requests timed (default 1100, ~3 s of arrivals); param_study: steps of "
+                   help="serving: timed rounds of one request per vector env (default 34: 1 122 requests, "
+                        "~2.8 s of arrivals); param_study: steps of "
                         "--periods-per-step (default: as many as cover 200 periods); mixed/large: ensemble "
                         "solves (default 1)")
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--rate", type=float, default=200000.0, help="serving: offered env-steps/s")
     p.add_argument("--saturation-s", type=float, default=3.0, help="serving: closed-loop seconds per path (0: off)")
+    p.add_argument("--poll-interval", type=int, default=0,
+                   help="serving: attempts between ring polls of a busy wave (0: the ServeOptions default)")
+    p.add_argument("--serving-plain", type=int, default=1,
+                   help="serving: also run the plain paths and the closed loops (0: the continuous path only, "
+                        "e.g. under a profiler)")
     p.add_argument("--periods-per-step", type=float, default=200.0,
                    help="param_study: periods integrated per step (one window-kernel launch)")
     p.add_argument("--side", type=int, default=256, help="param_study: PA grid side (systems = side^2)")
     p.add_argument("--env-bubbles", type=int, default=2, help="env: bubbles per env")
     p.add_argument("--env-count", type=int, default=16384, help="env: number of envs")
-    p.add_argument("--rollout-groups", type=int, default=4, help="rollout: groups of envs stepping on their own")
+    p.add_argument("--rollout-groups", type=int, default=4, help="rollout: blocks of envs stepping on their own")
     p.add_argument("--env-trajectories", type=int, default=0,
                    help="env: record trajectories (dense output, device arena, zarr files in a temporary "
                         "directory); 2 writes the files on a background thread")
@@ -139,9 +146,28 @@ def cpu_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
+def host_info() -> dict:
+    """CPU model and core counts of the host the CPU baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": usable, "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def roofline(flops: float, seconds: float, **extra) -> dict:
     achieved = flops / seconds / 1e12 if seconds > 0 else 0.0
-    out = {"bound": "mfma", "pipe": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+    out = {"bound": "fp64-valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
     out.update(extra)
     return out
@@ -154,14 +180,15 @@ def attempts_of(models) -> dict:
             for m in models}
 
 
-def cpu_step_sample(W, ws, target_s: float, label: str):
+def cpu_step_sample(W, ws, target_s: float, label: str, threads: int = 0):
     """C oracle (OpenMP) on growing subsets of the given workloads (the whole set at most,
-    then repeated) until one pass takes at least target_s / 4; returns the baseline dict."""
+    then repeated) until one pass takes at least target_s / 4; returns the baseline dict.
+    threads > 0 runs the oracle's loops on that many threads (1: one core)."""
     import numpy as np
 
     from oracle import ckm1d_oracle as O
 
-    nthreads = cpu_threads()
+    nthreads = O.set_threads(threads if threads > 0 else cpu_threads())
     frac, reps = 1.0 / 512, 1
     while True:
         periods, wall, systems = 0.0, 0.0, 0
@@ -185,8 +212,8 @@ def cpu_step_sample(W, ws, target_s: float, label: str):
         else:
             reps = int(np.ceil(reps * grow))
     return {"value": periods / wall, "unit": "periods/s", "cores": nthreads, "kind": "port",
-            "sample": f"{systems} system runs of {label} ({periods:.0f} periods), C oracle with OpenMP, "
-                      f"{wall:.1f} s"}
+            "sample": f"{systems} system runs of {label} ({periods:.0f} periods), C oracle "
+                      f"{'with OpenMP' if nthreads > 1 else 'on one core'}, {wall:.1f} s"}
 
 
 # ---------------------------------------------------------------------------
@@ -200,8 +227,16 @@ def bench_serving(args):
 
     sizes = {2: 12288, 4: 4096}
     K = 5.0
-    steps = args.steps or 1100
     venvs = vector_envs(sizes, np.random.default_rng(3))
+    V = len(venvs)
+    # A step is one round of the offered load: V requests, on average one env step of
+    # every vector env (16 384 env steps).  The request stream holds `warmup` rounds that
+    # load the queues, then the timed rounds, then COOL rounds that keep the load on
+    # while the timed requests finish; only the timed requests are measured.
+    steps = args.steps or 34
+    COOL = 3
+    n_req = (args.warmup + steps + COOL) * V
+    lo, hi = args.warmup * V, (args.warmup + steps) * V
     lam = W.wavelength(W.water_25c(), 25.0)
 
     def engine(mode, per_bin=False):
@@ -209,10 +244,12 @@ def bench_serving(args):
         bins = {n: W.build_model(w) for n, w in bins_w.items()}
         eng = ServingEngine(bins, periods=K, mode=mode, reset_prob=0.001,
                             reset_scene=lambda n, k, rng: c0_scene(n, k, rng, lam),
-                            serve_opts=ServeOptions(max_attempts=1 << 20), seed=3, plain_per_bin=per_bin)
+                            serve_opts=sopts, seed=3, plain_per_bin=per_bin)
         return eng, bins
 
-    def run(mode, reqs=None, closed=None, per_bin=False):
+    sopts = ServeOptions(max_attempts=1 << 20, **({"poll_interval": args.poll_interval} if args.poll_interval else {}))
+
+    def run(mode, reqs=None, closed=None, per_bin=False, window=True):
         eng, bins = engine(mode, per_bin)
         a0 = attempts_of(bins.values())
         gc.collect()
@@ -223,6 +260,7 @@ def bench_serving(args):
                 st = eng.run(first, timeout_s=300.0, progress=log, feeder=feeder)
             else:
                 st = eng.run(reqs, timeout_s=300.0, progress=log)
+            win = eng.window(lo, hi) if window and closed is None else None
         finally:
             gc.enable()
             eng.close()
@@ -230,68 +268,88 @@ def bench_serving(args):
         st["attempts"] = {n: a1[n] - a0[n] for n in a1}
         st["flops"] = float(sum(st["attempts"][n] * flops_per_attempt(n) for n in a1))
         info = {n: m.runtime.kernel.info() for n, m in bins.items()}
-        return st, info
+        return st, win, info
 
-    log(f"serving: {sum(sizes.values())} resident systems, {len(venvs)} vector envs, warm-up")
-    for _ in range(args.warmup):
-        run("continuous", make_requests(venvs, args.rate, count=60, seed=100))
+    log(f"serving: {sum(sizes.values())} resident systems, {V} vector envs; a step = {V} requests; "
+        f"{args.warmup} + {steps} + {COOL} rounds")
+    run("continuous", make_requests(venvs, args.rate, count=V, seed=100), window=False)   # module load, first launch
     torch.cuda.synchronize()
-    log(f"continuous: {steps} requests at {args.rate:g} env-steps/s")
-    cont, info = run("continuous", make_requests(venvs, args.rate, count=steps, seed=3))
-    log(f"continuous: {cont['periods_per_s']:.4g} periods/s p50 {cont['p50_ms']:.2f} ms p99 {cont['p99_ms']:.2f} ms")
-    log("plain (static batches): same requests")
-    plain, _ = run("plain", make_requests(venvs, args.rate, count=steps, seed=3))
-    log(f"plain: {plain['periods_per_s']:.4g} periods/s p50 {plain['p50_ms']:.2f} ms p99 {plain['p99_ms']:.2f} ms")
-    log("plain, one batch loop per bubble count: same requests")
-    plain_bin, _ = run("plain", make_requests(venvs, args.rate, count=steps, seed=3), per_bin=True)
-    log(f"plain per bin: {plain_bin['periods_per_s']:.4g} periods/s p50 {plain_bin['p50_ms']:.2f} ms "
-        f"p99 {plain_bin['p99_ms']:.2f} ms")
-    sat = None
-    if args.saturation_s > 0:
-        log(f"closed loop {args.saturation_s:g} s: continuous")
-        sc, _ = run("continuous", closed=args.saturation_s)
-        log(f"closed loop {args.saturation_s:g} s: plain")
-        sp, _ = run("plain", closed=args.saturation_s)
-        sat = {"continuous_periods_per_s": sc["periods_per_s"], "plain_periods_per_s": sp["periods_per_s"],
-               "ratio": sc["periods_per_s"] / sp["periods_per_s"],
-               "continuous_p50_ms": sc["p50_ms"], "continuous_p99_ms": sc["p99_ms"],
-               "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
-               "continuous_tflops": sc["flops"] / sc["wall_s"] / 1e12}
+    reqs = lambda: make_requests(venvs, args.rate, count=n_req, seed=3)   # noqa: E731
+    log(f"continuous: {n_req} requests at {args.rate:g} env-steps/s")
+    whole, cont, info = run("continuous", reqs())
+    log(f"continuous window: {cont['periods_per_s']:.4g} periods/s p50 {cont['p50_ms']:.2f} ms "
+        f"p99 {cont['p99_ms']:.2f} ms over {cont['requests']} requests")
+    plain = plain_bin = sat = None
+    if args.serving_plain:
+        log("plain (static batches): same requests")
+        _, plain, _ = run("plain", reqs())
+        log(f"plain window: {plain['periods_per_s']:.4g} periods/s p50 {plain['p50_ms']:.2f} ms "
+            f"p99 {plain['p99_ms']:.2f} ms")
+        log("plain, one batch loop per bubble count: same requests")
+        _, plain_bin, _ = run("plain", reqs(), per_bin=True)
+        log(f"plain per bin window: {plain_bin['periods_per_s']:.4g} periods/s p50 {plain_bin['p50_ms']:.2f} ms")
+        if args.saturation_s > 0:
+            log(f"closed loop {args.saturation_s:g} s: continuous")
+            sc, _, _ = run("continuous", closed=args.saturation_s)
+            log(f"closed loop {args.saturation_s:g} s: plain")
+            sp, _, _ = run("plain", closed=args.saturation_s)
+            sat = {"continuous_periods_per_s": sc["periods_per_s"], "plain_periods_per_s": sp["periods_per_s"],
+                   "ratio": sc["periods_per_s"] / sp["periods_per_s"],
+                   "continuous_p50_ms": sc["p50_ms"], "continuous_p99_ms": sc["p99_ms"],
+                   "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
+                   "continuous_tflops": sc["flops"] / sc["wall_s"] / 1e12,
+                   "continuous_frac": sc["flops"] / sc["wall_s"] / 1e12 / PEAK_FP64_TFLOPS}
 
-    cpu = None
+    cpu = cpu1 = None
     if args.cpu_baseline:
-        log("cpu baseline")
         ws = list(W.serving_bins(seed=3, sizes=sizes, periods=K).values())
+        log("cpu baseline (all cores)")
         cpu = cpu_step_sample(W, ws, args.cpu_seconds, "the resident set, one 5-period step each")
+        log("cpu baseline (one core)")
+        cpu1 = cpu_step_sample(W, ws, args.cpu_seconds * 0.6, "the resident set, one 5-period step each",
+                               threads=1)
+        cpu["one_core"] = {k: cpu1[k] for k in ("value", "cores", "sample")}
+        cpu["host"] = host_info()
     keys = ("wall_s", "requests", "env_steps", "periods", "periods_per_s", "env_steps_per_s", "p50_ms", "p99_ms",
-            "mean_ms", "max_ms", "failed_env_steps", "resets")
+            "mean_ms", "max_ms", "failed_env_steps")
+    att = cont["attempts"]
+    flops = float(sum(att[n] * flops_per_attempt(n) for n in att))
+    traffic = (SERVING_TRAFFIC_BYTES_PER_ENV_STEP * cont["env_steps"]
+               if args.rate == 200000.0 and SERVING_TRAFFIC_BYTES_PER_ENV_STEP else None)
     return {
         "value": cont["periods_per_s"],
         "unit": "periods/s",
         "steps": steps,
         "ms_per_step": cont["wall_s"] * 1e3 / steps,
         "config": {"workload": "serving_16384_N2x12288_N4x4096", "resident_systems": 16384,
-                   "bins": {"2": 12288, "4": 4096}, "vector_envs": len(venvs), "burst_envs": [64, 1024],
+                   "bins": {"2": 12288, "4": 4096}, "vector_envs": V, "burst_envs": [64, 1024],
                    "offered_env_steps_per_s": args.rate, "periods_per_step": K, "rtol": 1e-9, "atol": 1e-9,
                    "field": "SW_N", "drive_khz": [25.0, 50.0], "reset": "failed envs + 10% per 100 steps",
+                   "step": f"one round of {V} requests (every vector env once on average, 16 384 env steps)",
+                   "window": f"requests {lo}..{hi - 1} of {n_req}: after {args.warmup} warm-up rounds, "
+                             f"followed by {COOL} rounds that keep the load on",
                    "mapping": {str(n): i.mapping for n, i in info.items()},
-                   "vgprs": {str(n): i.num_regs for n, i in info.items()}},
-        "latency_ms": {"p50": cont["p50_ms"], "p99": cont["p99_ms"], "plain_p50": plain["p50_ms"],
-                       "plain_p99": plain["p99_ms"]},
-        "vs_plain": cont["periods_per_s"] / plain["periods_per_s"],
+                   "vgprs": {str(n): i.num_regs for n, i in info.items()},
+                   "poll_interval": sopts.poll_interval, "idle_us": sopts.idle_us},
+        "latency_ms": {"p50": cont["p50_ms"], "p99": cont["p99_ms"],
+                       "samples": cont["requests"],
+                       **({"plain_p50": plain["p50_ms"], "plain_p99": plain["p99_ms"]} if plain else {})},
+        "vs_plain": cont["periods_per_s"] / plain["periods_per_s"] if plain else None,
         "env_steps_per_s": cont["env_steps_per_s"],
         "continuous": {k: cont[k] for k in keys},
-        "plain": {k: plain[k] for k in keys},
-        "plain_per_bin": {k: plain_bin[k] for k in keys},
+        "plain": {k: plain[k] for k in keys} if plain else None,
+        "plain_per_bin": {k: plain_bin[k] for k in keys} if plain_bin else None,
         "saturation": sat,
-        "roofline": roofline(cont["flops"], cont["wall_s"], window="timed region (persistent serving kernels)",
+        "whole_run": {"requests": whole["requests"], "env_steps": whole["env_steps"], "wall_s": whole["wall_s"],
+                      "kernel_launches": whole.get("kernel_launches"),
+                      "attempts": {str(n): v for n, v in whole["attempts"].items()}},
+        "roofline": roofline(flops, cont["wall_s"], window="the timed requests' span (persistent serving kernels)",
                              flops_per_attempt={str(n): flops_per_attempt(n) for n in sizes},
-                             attempts={str(n): v for n, v in cont["attempts"].items()},
-                             traffic=(SERVING_TRAFFIC_BYTES_1100 if steps == 1100 and args.rate == 200000.0
-                                      else None),
-                             traffic_note="bytes of both serving kernels over the timed region from rocprofv3 "
-                                          "FETCH_SIZE/WRITE_SIZE of the default run, mostly ring polls to mapped "
-                                          "host memory (an upper bound on HBM bytes)"),
+                             attempts={str(n): v for n, v in att.items()},
+                             traffic=traffic,
+                             traffic_note="rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of both serving kernels "
+                                          "over a default run, per env step served, times the window's env steps; "
+                                          "mostly ring polls of mapped host memory (an upper bound on HBM bytes)"),
         "cpu_baseline": cpu,
     }
 
@@ -688,97 +746,111 @@ def bench_env(args):
 
 
 def bench_rollout(args):
+    """PPO-style rollout collection over PosNBC1D's full step (rewards, terminations,
+    time-outs, episode resets): the synchronous loop (policy on every env, then
+    PosNBC1D.step on every env, as rl/ppo.py:249-300) against ServedPosNBC1D with
+    --rollout-groups blocks stepping on their own over the serving kernel."""
     import numpy as np
     import torch
 
-    from coupledbubble_control import workloads as W
-    from coupledbubble_control.envs import ServedVecEnv
-    from coupledbubble_control.rollout import OverlappedRollout
+    from coupledbubble_control.envs import PosNBC1D, ServedPosNBC1D
 
-    n, nk, K = 2, 2, 5.0
+    n = args.env_bubbles
     ne = args.env_count
     steps = args.steps or 20
-    lam = W.wavelength(W.water_25c(), 25.0)
+    G = args.rollout_groups
     torch.manual_seed(0)
 
+    def make_env():
+        return PosNBC1D(num_bubbles=n, num_envs=ne, envs_per_block=64, verbose=False, collect_trajectories=False,
+                        episode_length=args.env_episode_length)
+
+    probe = make_env()
+    A, D = probe.action_space.space_data.shape[0], probe.observation_space.space_data.shape[0]
+    T = float(probe.time_step_length)
+    del probe
+
     def mlp(out):
-        return torch.nn.Sequential(torch.nn.Linear(n, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
+        return torch.nn.Sequential(torch.nn.Linear(D, 256), torch.nn.Tanh(), torch.nn.Linear(256, 256),
                                    torch.nn.Tanh(), torch.nn.Linear(256, out)).cuda()
-    actor = mlp(2 * nk)
-    policy_s = [0.0]
+    actor = mlp(2 * A)
     calls = [0]
 
-    def policy(o, ids):
+    def policy(obs, begin=0):
         calls[0] += 1
-        c0 = time.perf_counter()
         with torch.no_grad():
-            ab = torch.nn.functional.softplus(actor(torch.from_numpy(o).cuda())) + 1.0
-            pa = (1.5 * ab[:, :nk] / (ab[:, :nk] + ab[:, nk:])).double().cpu().numpy()
-        policy_s[0] += time.perf_counter() - c0
-        return pa
+            ab = torch.nn.functional.softplus(actor(obs)) + 1.0
+            return ab[:, :A] / (ab[:, :A] + ab[:, A:])
 
-    def run(groups):
-        w = W.serving_bins(seed=3, sizes={n: ne}, periods=K)[n]
-        gap = np.full(1, 3.0 * float(w.R0.max()) * 1e-6 / lam)
+    def synchronous():
+        env = make_env()
+        obs, _ = env.reset()
+        for _ in range(max(args.warmup, 1)):
+            obs = env.step(policy(obs))[0]
+        a0 = attempts_of([env.model])[n]
+        calls[0] = 0
+        torch.cuda.synchronize()
+        gc.collect()
+        gc.disable()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            obs = env.step(policy(obs))[0]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gc.enable()
+        return {"wall_s": wall, "attempts": attempts_of([env.model])[n] - a0, "policy_calls": calls[0]}
 
-        def scene(k, rng):
-            return W.sample_positions(rng, k, n, 0.0, 0.25, np.repeat(gap, k))
-        m = W.build_model(w)
-        env = ServedVecEnv(m, time_step_length=K, scene=scene, seed=5)
-        ro = OverlappedRollout(env, policy, groups=groups, seed=7)
+    def served(groups, lockstep=False):
+        env = make_env()
+        env.reset()
+        sv = ServedPosNBC1D(env, groups=groups)
         try:
-            obs0 = m.runtime.host_buffers.state.actual_state[1, :, :n].astype(np.float32)
-            obs1 = ro.collect(obs0, max(args.warmup, 1))["observations"][-1].copy()
-            a0 = attempts_of([m])[n]
-            policy_s[0] = 0.0
+            sv.rollout(policy, max(args.warmup, 1), lockstep=lockstep, record=False)
+            a0 = attempts_of([env.model])[n]
             calls[0] = 0
             gc.collect()
-            gc.disable()   # no collector pass inside the timed loop (as timeit does)
-            t0 = time.perf_counter()
-            r = ro.collect(obs1, steps)
-            wall = time.perf_counter() - t0
+            gc.disable()
+            r = sv.rollout(policy, steps, lockstep=lockstep, record=False)
             gc.enable()
-            att = attempts_of([m])[n] - a0
+            att = attempts_of([env.model])[n] - a0
         finally:
-            env.close()
-        out = {"wall_s": wall, "attempts": att, "policy_s": policy_s[0], "policy_calls": calls[0],
-               "failures": int(r["failures"].sum()), "group_done_s": [round(float(x), 4) for x in r["group_done_s"]]}
-        if "env_done_s" in r:
-            out["env_done_s_p50_max"] = [round(float(np.median(r["env_done_s"])), 4),
-                                         round(float(r["env_done_s"].max()), 4)]
-        return out
+            sv.close()
+        return {"wall_s": r["wall_s"], "attempts": att, "policy_calls": calls[0],
+                "group_done_s": [round(float(x), 4) for x in r["group_done_s"]]}
 
-    G = args.rollout_groups
-    log(f"rollout: {ne} envs x {steps} steps, synchronous loop")
-    sync = run(1)
-    log(f"rollout: {G} groups")
-    grp = run(G)
-    log("rollout: every env on its own")
-    per = run("env")
-    periods = ne * steps * K
+    log(f"rollout: {ne} envs x {n} bubbles x {steps} steps, synchronous PosNBC1D.step loop")
+    sync = synchronous()
+    log(f"rollout: served, {G} blocks on their own")
+    grp = served(G)
+    log("rollout: served, one block")
+    one = served(1)
+    periods = ne * steps * T
     cpu = None
     if args.cpu_baseline:
-        log("cpu baseline")
-        cpu = cpu_step_sample(W, [W.serving_bins(seed=3, sizes={n: ne}, periods=K)[n]], args.cpu_seconds,
-                              "the resident envs, one 5-period step each")
+        log("env: CPU baseline (oracle environment)")
+        cpu = _env_oracle_sample(n, T, args.cpu_seconds)
     F = flops_per_attempt(n)
+    best = min((sync, "synchronous"), (grp, f"served_{G}_blocks"), (one, "served_1_block"),
+               key=lambda x: x[0]["wall_s"])
     return {
-        "value": periods / grp["wall_s"],
+        "value": periods / sync["wall_s"],
         "unit": "periods/s",
         "steps": steps,
-        "ms_per_step": grp["wall_s"] * 1e3 / steps,
-        "config": {"workload": f"rollout: {ne} served envs x {n} bubbles, {K:g} periods per step, actor MLP "
-                               f"policy, {G} groups", "envs": ne, "bubbles": n, "periods_per_step": K,
-                   "groups": G, "policy": "actor MLP 256x256 tanh, Beta mean, on the observed positions",
-                   "periods": "nominal: env steps x 5 (failed envs restart at the next step)"},
-        "env_steps_per_s": ne * steps / grp["wall_s"],
+        "ms_per_step": sync["wall_s"] * 1e3 / steps,
+        "config": {"workload": f"rollout: PosNBC1D {ne} envs x {n} bubbles, {T:g} periods per step, actor MLP "
+                               f"policy; the synchronous loop, beside {G} served blocks and one",
+                   "envs": ne, "bubbles": n, "periods_per_step": T,
+                   "groups": G, "policy": "actor MLP 256x256 tanh, Beta mean, on the observations",
+                   "episode_length": args.env_episode_length},
+        "env_steps_per_s": ne * steps / sync["wall_s"],
         "synchronous": {**sync, "periods_per_s": periods / sync["wall_s"], "ms_per_step": sync["wall_s"] * 1e3 / steps},
-        "grouped": {**grp, "periods_per_s": periods / grp["wall_s"]},
-        "per_env": {**per, "policy_min_batch": max(1, ne // 16), "periods_per_s": periods / per["wall_s"],
-                    "ms_per_step": per["wall_s"] * 1e3 / steps, "vs_synchronous": sync["wall_s"] / per["wall_s"]},
+        "served_blocks": {**grp, "periods_per_s": periods / grp["wall_s"], "ms_per_step": grp["wall_s"] * 1e3 / steps},
+        "served_one_block": {**one, "periods_per_s": periods / one["wall_s"],
+                             "ms_per_step": one["wall_s"] * 1e3 / steps},
         "vs_synchronous": sync["wall_s"] / grp["wall_s"],
-        "roofline": roofline(grp["attempts"] * F, grp["wall_s"], window="timed region (persistent serving kernel)",
-                             flops_per_attempt=F, attempts=int(grp["attempts"])),
+        "fastest": best[1],
+        "roofline": roofline(sync["attempts"] * F, sync["wall_s"], window="timed synchronous rollout",
+                             flops_per_attempt=F, attempts=int(sync["attempts"])),
         "cpu_baseline": cpu,
     }
 

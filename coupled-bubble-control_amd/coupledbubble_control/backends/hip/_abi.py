@@ -16,7 +16,7 @@ import torch  # noqa: F401  (load torch's HIP runtime before libcbc)
 _PKG_ROOT = Path(__file__).resolve().parents[3]
 LIB_PATH = Path(os.environ.get("CBC_LIB", _PKG_ROOT / "lib" / "libcbc.so"))
 
-CBC_ABI_VERSION = 3
+CBC_ABI_VERSION = 4
 
 CBC_OK = 0
 CBC_ERR_INVALID = -1
@@ -48,6 +48,7 @@ CBC_EVENT_EXACT = 1
 EVENT_MODES = {"bracket": CBC_EVENT_BRACKET, "exact": CBC_EVENT_EXACT}
 
 CBC_SERVE_RESET = 1
+CBC_SERVE_USE_CONTROL = 2
 
 MATH_CODES = {"sincos": 0, "exp": 1, "log": 2, "pow": 3, "rcp": 4, "sincos_kern": 5, "sincos_tiny": 6,
               "rotate": 7, "pow1p": 8}
@@ -126,7 +127,8 @@ class EnvSpecC(ctypes.Structure):
                 ("apply_termination", _i32), ("episode_length", _i64), ("position_tolerance", ctypes.c_float),
                 ("r_dnorm", ctypes.c_float), ("reward_exp", ctypes.c_float), ("w_distance", ctypes.c_float),
                 ("w_intensity", ctypes.c_float), ("positive_terminal_reward", ctypes.c_float),
-                ("negative_terminal_reward", ctypes.c_float), ("next_time_step", _f64)]
+                ("negative_terminal_reward", ctypes.c_float), ("next_time_step", _f64),
+                ("env_begin", _i32), ("env_count", _i32)]
 
 
 class EnvBuffersC(ctypes.Structure):

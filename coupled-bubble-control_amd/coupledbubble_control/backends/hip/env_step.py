@@ -173,6 +173,48 @@ class EnvKernels:
         self.observations = out.observations
         return counts
 
+    def _range_spec(self, begin: int, count: int) -> _abi.EnvSpecC:
+        if not (0 <= begin and count >= 1 and begin + count <= self.ne):
+            raise ValueError(f"env range [{begin}, {begin + count}) outside [0, {self.ne})")
+        s = _abi.EnvSpecC.from_buffer_copy(self.spec)
+        s.env_begin, s.env_count = int(begin), int(count)
+        return s
+
+    def set_action_range(self, begin: int, actions: torch.Tensor, stream=None) -> None:
+        """set_action for envs [begin, begin + len(actions)) only (asynchronous)."""
+        a = torch.as_tensor(actions)
+        count = int(a.shape[0])
+        if tuple(a.shape) != (count, self.spec.num_actions):
+            raise ValueError(f"actions must have shape (n, {self.spec.num_actions}), got {tuple(a.shape)}")
+        spec = self._range_spec(begin, count)
+        self.actions[begin:begin + count].copy_(a.to(device=self.device, dtype=torch.float32))
+        self._check_solver_buffers()
+        b = self._buffers(None)
+        _abi.check(_abi.lib().cbc_env_set_action(ctypes.byref(spec), ctypes.byref(b), _abi.stream_handle(stream)))
+
+    def new_outputs(self) -> StepOutputs:
+        """Output tensors for every env (written row range by row range by post_range)."""
+        dev = self.device
+        return StepOutputs(
+            observations=torch.zeros((self.ne, self.obs_dim), dtype=torch.float32, device=dev),
+            rewards=torch.zeros(self.ne, dtype=torch.float32, device=dev),
+            failure=torch.zeros(self.ne, dtype=torch.bool, device=dev),
+            success=torch.zeros(self.ne, dtype=torch.bool, device=dev),
+            terminated=torch.zeros(self.ne, dtype=torch.bool, device=dev),
+            time_outs=torch.zeros(self.ne, dtype=torch.bool, device=dev))
+
+    def post_range(self, begin: int, count: int, observed: int, out: StepOutputs, stream=None) -> _abi.EnvCountsC:
+        """observe .. advance_time for envs [begin, begin + count) into the rows of ``out``
+        (``observed``: the posts this range has had); the done list holds the range's
+        done envs (global ids, ascending).  Returns the counts (synchronises the stream)."""
+        spec = self._range_spec(begin, count)
+        self._check_solver_buffers()
+        b = self._buffers(out)
+        counts = _abi.EnvCountsC()
+        _abi.check(_abi.lib().cbc_env_post(ctypes.byref(spec), ctypes.byref(b), int(observed),
+                                           _abi.stream_handle(stream), ctypes.byref(counts)))
+        return counts
+
     def reset(self, scene: _abi.EnvSceneC, ids: torch.Tensor, observed: int, final: bool, stream=None,
               observations: Optional[torch.Tensor] = None):
         """Final outputs of ``ids`` (when ``final``) then their reset (:583-614,

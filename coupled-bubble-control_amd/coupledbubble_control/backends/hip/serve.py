@@ -104,22 +104,32 @@ class ServeQueue:
         self.stage_up[:, sys] = unit_params
         self.stage_cm[:, :, sys] = coupling
 
-    def submit(self, sys: np.ndarray, pa: np.ndarray, t_end: float, h0=0.0,
-               reset: Optional[np.ndarray] = None) -> Optional[np.ndarray]:
+    def submit(self, sys: np.ndarray, pa: Optional[np.ndarray], t_end: float, h0=0.0,
+               reset: Optional[np.ndarray] = None, use_control: bool = False) -> Optional[np.ndarray]:
         """Publish one step for systems ``sys`` with amplitudes ``pa`` (NK, n) [Pa]; h0 > 0
         (scalar or per item) replaces the resident step size; ``reset`` (bool per item)
-        starts new episodes from the staged rows.  Returns the items' sequence numbers, or
+        starts new episodes from the staged rows; ``use_control`` takes the amplitudes from
+        the device's control_params instead (``pa`` may then be None), where the
+        environment's set_action has written them.  Returns the items' sequence numbers, or
         None when the ring has too few free slots (nothing published)."""
         sys = np.ascontiguousarray(sys, dtype=np.int32)
         n = sys.size
+        if pa is None:
+            if not use_control:
+                raise ValueError("amplitudes are needed unless use_control is set")
+            pa = np.zeros((self.nk, n))
         pa = np.ascontiguousarray(pa, dtype=np.float64).reshape(self.nk, n)
         if self._wmax * float(t_end) + self._phmax >= _MAX_TRIG_ARG:
             raise ValueError("t_end too long for the device sincos range")
         h = np.ascontiguousarray(np.broadcast_to(np.asarray(h0, dtype=np.float64), (n,)))
         opts = None
-        if reset is not None:
-            opts = np.ascontiguousarray(np.where(np.asarray(reset, dtype=bool), _abi.CBC_SERVE_RESET, 0),
-                                        dtype=np.int32)
+        if reset is not None or use_control:
+            bits = np.zeros(n, dtype=np.int32)
+            if reset is not None:
+                bits |= np.where(np.asarray(reset, dtype=bool), _abi.CBC_SERVE_RESET, 0).astype(np.int32)
+            if use_control:
+                bits |= _abi.CBC_SERVE_USE_CONTROL
+            opts = np.ascontiguousarray(bits)
         seq = np.empty(n, dtype=np.uint32)
         rc = _abi.lib().cbc_serve_submit(self.handle, sys.ctypes.data, pa.ctypes.data, int(n), float(t_end),
                                          h.ctypes.data, opts.ctypes.data if opts is not None else None,

@@ -15,7 +15,7 @@ Reward shaping, targets and observation stacking stay with the environment.
 """
 from __future__ import annotations
 
-from typing import Callable, Optional, Tuple
+from typing import Callable, Dict, Optional, Tuple
 
 import numpy as np
 
@@ -118,6 +118,188 @@ class ServedVecEnv:
         state, up, cm = self.model.scene_rows(env_ids, None, X0)
         self.queue.stage(env_ids, state, up, cm)
         self._reset[env_ids] = True
+
+    def close(self) -> None:
+        self.queue.close()
+
+
+class ServedPosNBC1D:
+    """``PosNBC1D.step`` (envs/bubble_1D_pos_control.py:503-538) served group by group.
+
+    The envs of a ``PosNBC1D`` are split into ``groups`` contiguous blocks that step on
+    their own.  A block's step is the reference step with the solver call replaced by
+    serving items: ``set_action`` on the block's rows (cbc_env_set_action with an env
+    range: amplitudes and phases into control_params), one item per env that takes its
+    amplitudes from control_params and integrates ``time_step_length`` from the resident
+    state with the step size ``advance_time`` left, then -- once the block's items have
+    finished -- ``observe``, ``check_termination``, ``get_rewards`` and ``advance_time`` on
+    the block's rows (cbc_env_post with an env range) and ``final_termination`` with
+    ``reset_envs`` for its done envs (cbc_env_reset, scenes from the block's own scene
+    generator).  While one block is being observed, reset and given its next actions,
+    the other blocks' items keep integrating on the serving kernel.
+
+    A system's arithmetic on the serving kernel equals the plain launch's, so with one
+    block a rollout equals ``PosNBC1D.step`` in a loop bit for bit (the block's scene
+    generator is the environment's, seeded alike); with several it equals stepping the
+    blocks in lockstep, whatever order they finish in.  Block g > 0 draws its scenes
+    from a generator seeded ``seed + 1000 g``.  Trajectory recording is not served
+    (the serving kernels record no dense output)."""
+
+    def __init__(self, env, groups: int = 1, opts=None, timeout_s: float = 300.0):
+        import copy
+
+        import torch
+
+        from ..backends.hip.serve import ServeOptions, ServeQueue
+
+        if env.trajectory_buffer is not None:
+            raise ValueError("served steps record no trajectories: build the env with collect_trajectories=False")
+        if env._observations is None:
+            raise RuntimeError("call env.reset() first")
+        ne = env.num_envs
+        if not isinstance(groups, (int, np.integer)) or not 1 <= groups <= ne:
+            raise ValueError(f"groups must be in [1, {ne}], got {groups}")
+        self.env = env
+        self.kernels = env._kernels
+        self.num_envs = ne
+        self.T = float(env.time_step_length)
+        self.timeout_s = float(timeout_s)
+        self.blocks = [(int(b[0]), int(b.size)) for b in np.array_split(np.arange(ne), groups)]
+        self.ids = [np.arange(b, b + c, dtype=np.int32) for b, c in self.blocks]
+        self.scenes = []
+        for g in range(groups):
+            sg = copy.copy(env.scene_generator)
+            sg._generators = {}
+            sg._seed = env.scene_generator._seed + 1000 * g
+            sg.last = None
+            self.scenes.append(sg)
+        self.observed = [env._observed] * groups
+        cap = 64
+        while cap < 2 * ne:
+            cap *= 2
+        o = opts or ServeOptions()
+        self.queue = ServeQueue(env.model, ServeOptions(**{**o.__dict__, "capacity": max(o.capacity, cap)}))
+        self.out = self.kernels.new_outputs()
+        self.out.observations.copy_(env._observations)
+        self.torch = torch
+        self.draw_log: Optional[list] = None     # (group, step, done ids, draws) when recording
+
+    # -- one block ---------------------------------------------------------------
+    def _start(self, g: int, actions):
+        b, c = self.blocks[g]
+        self.kernels.set_action_range(b, actions)
+        self.torch.cuda.current_stream().synchronize()     # control rows written before the items
+        return self.queue.submit(self.ids[g], None, self.T, 0.0, use_control=True)
+
+    def _finish(self, g: int, step: int):
+        """observe .. reset of block g's finished step; returns (counts, done ids, final outputs)."""
+        b, c = self.blocks[g]
+        counts = self.kernels.post_range(b, c, self.observed[g], self.out)
+        self.observed[g] += 1
+        k = int(counts.done)
+        if k == 0:
+            return counts, None, None
+        ids = self.kernels.done_ids[:k].clone()
+        d = self.scenes[g].draw(k, self.torch.float64, self.env.device, self.kernels.device)
+        if self.draw_log is not None:
+            self.draw_log.append((g, step, ids.cpu().numpy(), {n: v.cpu().numpy() for n, v in d.draws.items()}))
+        fin = self.kernels.reset(d.scene, ids, self.observed[g], final=True, observations=self.out.observations)
+        return counts, ids, fin
+
+    # -- rollouts -----------------------------------------------------------------
+    def rollout(self, policy, steps: int, lockstep: bool = False, record: bool = True) -> Dict:
+        """``steps`` env steps of every env.  ``policy(obs, begin)`` maps a block's
+        observation rows (n, obs_dim) float32 on the device to its actions (n, A).  Blocks
+        step on their own (``lockstep=False``) or all together (the synchronous loop of
+        rl/ppo.py:249-300: every block steps, then the policy runs on every block).
+        Returns observations (steps + 1, NE, obs_dim), actions (steps, NE, A), rewards,
+        terminated, time_outs (steps, NE) as CPU tensors when ``record``, the done ids
+        per (step, block) and the seconds at which each block finished."""
+        import time
+
+        torch = self.torch
+        ne, G = self.num_envs, len(self.blocks)
+        obs_dim = self.kernels.obs_dim
+        A = self.kernels.spec.num_actions
+        rec = None
+        if record:
+            rec = {"observations": torch.zeros((steps + 1, ne, obs_dim)),
+                   "actions": torch.zeros((steps, ne, A)), "rewards": torch.zeros((steps, ne)),
+                   "terminated": torch.zeros((steps, ne), dtype=torch.bool),
+                   "time_outs": torch.zeros((steps, ne), dtype=torch.bool)}
+            rec["observations"][0] = self.out.observations.cpu()
+        dones: Dict = {}
+        t = [0] * G
+        seq = [None] * G
+        pending = [None] * G             # actions waiting for ring slots
+        done_at = np.zeros(G)
+        t0 = time.perf_counter()
+
+        def act_and_submit(g):
+            b, c = self.blocks[g]
+            if pending[g] is None:
+                a = policy(self.out.observations[b:b + c], b)
+                if rec is not None:
+                    rec["actions"][t[g], b:b + c] = a.detach().float().cpu()
+                pending[g] = a
+            s = self._start(g, pending[g])
+            if s is not None:
+                seq[g] = s
+                pending[g] = None
+
+        def finish(g):
+            b, c = self.blocks[g]
+            counts, ids, fin = self._finish(g, t[g])
+            if rec is not None:
+                o = self.out
+                rec["rewards"][t[g], b:b + c] = o.rewards[b:b + c].cpu()
+                rec["terminated"][t[g], b:b + c] = o.terminated[b:b + c].cpu()
+                rec["time_outs"][t[g], b:b + c] = o.time_outs[b:b + c].cpu()
+                rec["observations"][t[g] + 1, b:b + c] = o.observations[b:b + c].cpu()
+            if ids is not None:
+                dones[(t[g], g)] = (ids.cpu(), [x.cpu() for x in fin])
+            seq[g] = None
+            t[g] += 1
+            if t[g] == steps:
+                done_at[g] = time.perf_counter() - t0
+
+        if steps > 0:
+            for g in range(G):
+                act_and_submit(g)
+        while min(t) < steps:
+            if time.perf_counter() - t0 > self.timeout_s:
+                raise TimeoutError(f"rollout did not finish in {self.timeout_s} s (steps done {t})")
+            progressed = False
+            if lockstep:
+                if all(seq[g] is not None and self.queue.done(self.ids[g], seq[g]).all() for g in range(G)):
+                    for g in range(G):
+                        finish(g)
+                    if min(t) < steps:
+                        for g in range(G):
+                            act_and_submit(g)
+                    progressed = True
+                elif any(seq[g] is None for g in range(G)):
+                    for g in range(G):
+                        if seq[g] is None:
+                            act_and_submit(g)
+            else:
+                for g in range(G):
+                    if t[g] >= steps:
+                        continue
+                    if seq[g] is None:
+                        act_and_submit(g)
+                    elif self.queue.done(self.ids[g], seq[g]).all():
+                        finish(g)
+                        progressed = True
+                        if t[g] < steps:
+                            act_and_submit(g)
+            if not progressed:
+                self.queue.poll()
+                time.sleep(5e-5)
+        out = {"dones": dones, "group_done_s": done_at, "wall_s": time.perf_counter() - t0}
+        if rec is not None:
+            out.update(rec)
+        return out
 
     def close(self) -> None:
         self.queue.close()

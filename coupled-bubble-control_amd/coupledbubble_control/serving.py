@@ -65,6 +65,7 @@ class Request:
     flags: Optional[np.ndarray] = None
     periods: float = 0.0
     seq: Optional[np.ndarray] = None
+    attempts: int = -1              # step attempts of its items (continuous mode; -1 unknown)
 
     @property
     def latency(self) -> float:
@@ -268,6 +269,7 @@ class ServingEngine:
             q = self.queues[r.n]
             if q.done(r.env_ids, r.seq).all():
                 _, misc = q.results(r.env_ids)
+                r.attempts = int(misc[:, 3].sum())
                 self._finish(r, misc[:, 2], misc[:, 0], now)
                 n_done += 1
             else:
@@ -394,6 +396,31 @@ class ServingEngine:
             out["ring_full_retries"] = self.ring_full
             out["kernel_launches"] = int(sum(q.stats()[0] for q in self.queues.values()))
         return out
+
+    def window(self, lo: int, hi: int) -> Dict[str, float]:
+        """Statistics of the requests with id in [lo, hi) (a steady-state window: requests
+        before lo load the queues, requests after hi keep them loaded while the window's
+        requests finish).  The window runs from the first of its requests' arrivals to the
+        last of their completions; throughput is their integrated periods over that span,
+        latency is per request (arrival to completion)."""
+        rs = [r for r in self.completed if lo <= r.rid < hi]
+        if len(rs) != hi - lo:
+            raise ValueError(f"window [{lo}, {hi}) has {len(rs)} completed requests")
+        t0 = min(r.arrival for r in rs)
+        t1 = max(r.done for r in rs)
+        wall = t1 - t0
+        lat = np.array([r.latency for r in rs]) * 1e3
+        periods = sum(r.periods for r in rs)
+        env_steps = sum(r.size for r in rs)
+        att: Dict[int, int] = {}
+        for r in rs:
+            att[r.n] = att.get(r.n, 0) + max(r.attempts, 0)
+        return {"wall_s": wall, "requests": len(rs), "env_steps": env_steps, "periods": periods,
+                "periods_per_s": periods / wall, "env_steps_per_s": env_steps / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+                "failed_env_steps": sum(int((r.flags != 1).sum()) for r in rs),
+                "attempts": att if all(r.attempts >= 0 for r in rs) else None}
 
     def close(self) -> None:
         for q in self.queues.values():

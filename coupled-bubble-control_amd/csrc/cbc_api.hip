@@ -19,6 +19,9 @@
 #include "cbc_linalg.h"
 #include "cbc_math.h"
 
+static_assert(cbc::kServeReset == CBC_SERVE_RESET && cbc::kServeUseControl == CBC_SERVE_USE_CONTROL,
+              "item option bits of the ABI and the ring");
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -525,6 +528,7 @@ struct cbc_serve {
     int32_t waves;
     int32_t capacity;
     int32_t nk;
+    cbc::RingHost rh;            // ring slots, per-system sequence numbers (cbc_ring.h)
     cbc::ServeItem* ring;        // host-mapped, capacity entries
     uint32_t* ctl;               // host-mapped: [0] tail, [1] stop
     uint32_t* done_seq;          // host-mapped (NS)
@@ -534,8 +538,6 @@ struct cbc_serve {
     double* stage_up;
     double* stage_cm;
     uint32_t* last_seq;          // host (NS): seq of the last item submitted per system
-    uint32_t tail;               // items published
-    uint32_t retired;            // oldest ring slot not yet known finished
     int64_t launches;
 };
 
@@ -547,15 +549,7 @@ bool serve_running(cbc_serve* s)
     return hipEventQuery(s->ended) == hipErrorNotReady;
 }
 
-void serve_retire(cbc_serve* s)
-{
-    while (s->retired != s->tail) {
-        const cbc::ServeItem& it = s->ring[s->retired & (uint32_t)(s->capacity - 1)];
-        const uint32_t done = __atomic_load_n(&s->done_seq[it.sys], __ATOMIC_ACQUIRE);
-        if ((int32_t)(done - it.seq) < 0) break;
-        ++s->retired;
-    }
-}
+void serve_retire(cbc_serve* s) { cbc::ring_retire(s->rh); }
 
 int32_t serve_launch(cbc_serve* s)
 {
@@ -654,6 +648,14 @@ int32_t cbc_serve_create(cbc_plan* plan, const cbc_buffers* buf, const cbc_serve
             return hip_fail(e, "cbc_serve_create allocation");
         }
         std::memset(s->ring, 0, sizeof(cbc::ServeItem) * (size_t)cap);
+        s->rh.ring = s->ring;
+        s->rh.capacity = (uint32_t)cap;
+        s->rh.ns = plan->exec.ns;
+        s->rh.nk = plan->model.nk;
+        s->rh.done_seq = s->done_seq;
+        s->rh.last_seq = s->last_seq;
+        s->rh.tail = 0;
+        s->rh.retired = 0;
         std::memset(s->ctl, 0, 64);
         std::memset(s->done_seq, 0, sizeof(uint32_t) * ns);
         std::memset(s->out_misc, 0, sizeof(double) * 4 * ns);
@@ -755,44 +757,18 @@ int32_t cbc_serve_submit(cbc_serve* s, const int32_t* sys, const double* pa, int
     if (n < 0 || (n > 0 && (!sys || !pa))) return fail(CBC_ERR_INVALID, "cbc_serve_submit: bad item arrays");
     if (!(t_end > 0.0) || !(t_end < 1e300)) return fail(CBC_ERR_INVALID, "t_end must be positive and finite");
     if (n == 0) return CBC_OK;
-    serve_retire(s);
-    const uint32_t used = s->tail - s->retired;
-    if ((int64_t)used + n > (int64_t)s->capacity)
-        return fail(CBC_ERR_BUSY, "serving ring full: %u of %d slots in use, %d requested", used, s->capacity, n);
-    const int32_t ns = s->plan->exec.ns;
-    for (int32_t k = 0; k < n; ++k) {
-        const int32_t q = sys[k];
-        if (q < 0 || q >= ns) return fail(CBC_ERR_INVALID, "system index %d out of [0, %d)", q, ns);
-        const uint32_t done = __atomic_load_n(&s->done_seq[q], __ATOMIC_ACQUIRE);
-        if (done != s->last_seq[q])
-            return fail(CBC_ERR_INVALID, "system %d already has an unfinished item", q);
-        if (opts && (opts[k] & ~CBC_SERVE_RESET) != 0)
-            return fail(CBC_ERR_INVALID, "unknown item option bits 0x%x", (unsigned)opts[k]);
+    int32_t bad = 0;
+    switch (cbc::ring_submit(s->rh, sys, pa, n, t_end, h0, opts, seq_out, &bad)) {
+    case cbc::kRingOk: break;
+    case cbc::kRingFull:
+        return fail(CBC_ERR_BUSY, "serving ring full: %u of %d slots in use, %d requested", s->rh.tail - s->rh.retired,
+                    s->capacity, n);
+    case cbc::kRingBadSystem: return fail(CBC_ERR_INVALID, "system index %d out of [0, %d)", bad, s->rh.ns);
+    case cbc::kRingBusySystem: return fail(CBC_ERR_INVALID, "system %d already has an unfinished item", bad);
+    case cbc::kRingDuplicate: return fail(CBC_ERR_INVALID, "system %d appears twice in one submission", bad);
+    case cbc::kRingBadOptions: return fail(CBC_ERR_INVALID, "unknown item option bits 0x%x", (unsigned)bad);
     }
-    // duplicates within this submission: mark, then check
-    for (int32_t k = 0; k < n; ++k) {
-        const int32_t q = sys[k];
-        if (s->last_seq[q] != __atomic_load_n(&s->done_seq[q], __ATOMIC_ACQUIRE)) {
-            for (int32_t j = 0; j < k; ++j) --s->last_seq[sys[j]];
-            return fail(CBC_ERR_INVALID, "system %d appears twice in one submission", q);
-        }
-        ++s->last_seq[q];
-    }
-    for (int32_t k = 0; k < n; ++k) {
-        const int32_t q = sys[k];
-        cbc::ServeItem& it = s->ring[(s->tail + (uint32_t)k) & (uint32_t)(s->capacity - 1)];
-        it.sys = q;
-        it.seq = s->last_seq[q];
-        it.opts = opts ? opts[k] : 0;
-        it.pad = 0;
-        it.t_end = t_end;
-        it.h0 = h0 ? h0[k] : 0.0;
-        it.pa[0] = pa[k];
-        it.pa[1] = s->nk > 1 ? pa[(size_t)n + k] : 0.0;
-        if (seq_out) seq_out[k] = it.seq;
-    }
-    s->tail += (uint32_t)n;
-    __atomic_store_n(&s->ctl[0], s->tail, __ATOMIC_RELEASE);
+    __atomic_store_n(&s->ctl[0], s->rh.tail, __ATOMIC_RELEASE);
     if (!serve_running(s)) return serve_launch(s);
     return CBC_OK;
 }
@@ -801,13 +777,13 @@ int32_t cbc_serve_poll(cbc_serve* s, int32_t* pending)
 {
     if (!s) return fail(CBC_ERR_INVALID, "cbc_serve_poll: null handle");
     serve_retire(s);
-    const uint32_t left = s->tail - s->retired;
+    const uint32_t left = s->rh.tail - s->rh.retired;
     if (pending) *pending = (int32_t)left;
     if (left > 0 && !serve_running(s)) {
         // the kernel has ended (idle timeout) with items it never claimed;
         // retire again first: items it finished are visible once it has ended
         serve_retire(s);
-        if (s->tail != s->retired) return serve_launch(s);
+        if (s->rh.tail != s->rh.retired) return serve_launch(s);
     }
     return CBC_OK;
 }
@@ -816,8 +792,8 @@ int32_t cbc_serve_stats(const cbc_serve* s, int64_t* launches, uint32_t* publish
 {
     if (!s) return fail(CBC_ERR_INVALID, "cbc_serve_stats: null handle");
     if (launches) *launches = s->launches;
-    if (published) *published = s->tail;
-    if (retired) *retired = s->retired;
+    if (published) *published = s->rh.tail;
+    if (retired) *retired = s->rh.retired;
     return CBC_OK;
 }
 

@@ -173,10 +173,17 @@ __device__ __forceinline__ uint32_t popc64(uint64_t m) { return (uint32_t)__popc
 // ---------------------------------------------------------------------------
 // kernels
 
+// envs a launch covers: [env_begin, env_begin + env_count), all when env_count is 0
+__host__ __device__ __forceinline__ int range_count(const cbc_env_spec& s)
+{
+    return s.env_count > 0 ? s.env_count : s.num_envs;
+}
+
 __global__ __launch_bounds__(kBlock) void env_set_action(cbc_env_spec s, const float* actions, double* cp)
 {
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= s.num_envs) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= range_count(s)) return;
+    const int e = s.env_begin + k;
     const size_t NE = (size_t)s.num_envs;
     const float* a = actions + (size_t)e * s.num_actions;
     for (int c = 0; c < s.num_actions; ++c) cp[(size_t)s.action_row[c] * NE + e] = (double)a[c] * s.action_scale[c];
@@ -186,8 +193,9 @@ __global__ __launch_bounds__(kBlock) void env_post(PostArgs a)
 {
     const cbc_env_spec& s = a.s;
     const cbc_env_buffers& b = a.b;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    const bool live = e < s.num_envs;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k < range_count(s);
+    const int e = s.env_begin + k;
     bool f_fail = false, f_succ = false, f_tout = false;
     if (live) {
         const int N = s.num_bubbles;
@@ -403,6 +411,10 @@ int32_t check_spec(const cbc_env_spec* s)
     if (!s) return fail(CBC_ERR_INVALID, "env: null spec");
     if (s->num_envs < 1 || s->num_envs > kMaxEnvs)
         return fail(CBC_ERR_INVALID, "env: num_envs must be in [1, %d], got %d", kMaxEnvs, s->num_envs);
+    if (s->env_begin < 0 || s->env_count < 0 || (int64_t)s->env_begin + s->env_count > s->num_envs ||
+        (s->env_count == 0 && s->env_begin != 0))
+        return fail(CBC_ERR_INVALID, "env: range [%d, %d + %d) outside the %d envs (env_count 0 needs env_begin 0)",
+                    s->env_begin, s->env_begin, s->env_count, s->num_envs);
     if (s->num_bubbles < 1 || s->num_bubbles > CBC_ENV_MAX_BUBBLES)
         return fail(CBC_ERR_INVALID, "env: num_bubbles must be in [1, %d], got %d", CBC_ENV_MAX_BUBBLES,
                     s->num_bubbles);
@@ -511,12 +523,12 @@ int32_t env_post_run(const cbc_env_spec* spec, cbc_env_buffers b, uint32_t* ws, 
     b.workspace = ws;
     PostArgs a{*spec, b, observed};
     void* args[] = {&a};
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_post), dim3(n_blocks(spec->num_envs)),
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_post), dim3(n_blocks(range_count(*spec))),
                                    dim3(kBlock), args, 0, st);
     if (e != hipSuccess) return hip_fail(e, "env_post launch");
     {
         const uint32_t* cws = ws;
-        int nb = n_blocks(spec->num_envs);
+        int nb = n_blocks(range_count(*spec));
         int64_t* ids = b.done_ids;
         void* cargs[] = {&cws, &nb, &ids};
         e = hipLaunchKernel(reinterpret_cast<const void*>(&env_compact), dim3(nb), dim3(kBlock), cargs, 0, st);
@@ -555,7 +567,7 @@ int32_t cbc_env_set_action(const cbc_env_spec* spec, const cbc_env_buffers* buf,
     const float* act = buf->actions;
     double* cp = buf->control_params;
     void* args[] = {&s, &act, &cp};
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_set_action), dim3(n_blocks(s.num_envs)),
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&env_set_action), dim3(n_blocks(range_count(s))),
                                    dim3(kBlock), args, 0, (hipStream_t)stream);
     return e == hipSuccess ? CBC_OK : hip_fail(e, "env_set_action launch");
 }

@@ -96,7 +96,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         const double sp4 = a.sp[4 * NS + s];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            const double amps = it ? it->pa[k] : a.cp[(size_t)k * NS + s];
+            const double amps = (it && !(it->opts & kServeUseControl)) ? it->pa[k] : a.cp[(size_t)k * NS + s];
             const double omega = a.cp[(size_t)(NK + k) * NS + s];
             const double phase = a.cp[(size_t)(2 * NK + k) * NS + s];
             const double wave = a.cp[(size_t)(3 * NK + k) * NS + s];

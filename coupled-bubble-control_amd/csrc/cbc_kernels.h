@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cbc_ring.h"
+
 namespace cbc {
 
 // Arguments of one ensemble launch in the reference buffer layout.
@@ -71,27 +73,7 @@ struct LaunchArgs {
     const double* stage_cm;      // (3, UPS, NS, TILE)
 };
 
-// ServeItem.opts bits
-enum : int32_t {
-    kServeReset = 1,   // new episode: state, unit parameters and coupling rows come from the staging mirrors
-};
-
-// One env step of one system: integrate from the resident state (actual_state,
-// time_step) over tau in [0, t_end] with drive amplitudes pa[0..NK) [Pa]; the
-// stored event value is not consulted (the environment resets it every step).
-// h0 > 0 replaces the resident step size.  With kServeReset the system first
-// takes its state, unit parameters and coupling rows from the staging mirrors
-// and stores them into the device buffers.  48 bytes, written by the host only.
-struct ServeItem {
-    int32_t sys;
-    uint32_t seq;
-    int32_t opts;
-    int32_t pad;
-    double t_end;
-    double h0;
-    double pa[2];
-};
-static_assert(sizeof(ServeItem) == 48, "ServeItem layout");
+// ServeItem and its option bits: cbc_ring.h
 
 struct KernelAttr {
     int32_t num_regs;

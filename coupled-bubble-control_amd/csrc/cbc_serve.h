@@ -105,12 +105,17 @@ __device__ __forceinline__ void serve_publish_flag(const LaunchArgs& a, int s, u
 }
 
 // Idle handling of a wave with no live system.  Returns true when the wave
-// should exit.  `idle_since` is 0 while the wave is busy; the sleep between
-// polls of the host-mapped ring grows to about 50 us so idle waves do not
-// flood the host link.
+// should exit.  `idle_since` is 0 while the wave is busy.  Every idle round
+// reads the ring (tail and claim counter) before it gets here, and those reads
+// of host-mapped memory are the kernels' main traffic, so the sleep between
+// rounds grows to kMaxNaps x 127 x 64 clocks (~0.2 ms): with hundreds of idle
+// waves polling at random phases a new item is still seen within
+// microseconds.  The stop flag is read every 8th round (the host raises it
+// only at destroy).
+constexpr int kMaxNaps = 64;
+
 __device__ __forceinline__ bool serve_idle(const LaunchArgs& a, uint64_t& idle_since, int& idle_rounds)
 {
-    if (sys_load_u32(a.ring_stop) != 0u) return true;
     const uint64_t now = wall_clock64();
     if (idle_since == 0) {
         idle_since = now | 1ull;
@@ -118,7 +123,8 @@ __device__ __forceinline__ bool serve_idle(const LaunchArgs& a, uint64_t& idle_s
     } else if (now - idle_since > a.idle_ticks) {
         return true;
     }
-    const int naps = idle_rounds < 16 ? idle_rounds + 1 : 16;
+    if ((idle_rounds & 7) == 0 && sys_load_u32(a.ring_stop) != 0u) return true;
+    const int naps = idle_rounds < kMaxNaps ? idle_rounds + 1 : kMaxNaps;
     for (int k = 0; k < naps; ++k) __builtin_amdgcn_s_sleep(127);
     ++idle_rounds;
     return false;

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CBC_ABI_VERSION 3
+#define CBC_ABI_VERSION 4
 
 /* return codes */
 #define CBC_OK 0
@@ -258,6 +258,9 @@ typedef struct cbc_serve_view {
  * host filled before submitting, and stored into the device buffers
  * (actual_state, unit_params, coupling_matrices are then written). */
 #define CBC_SERVE_RESET 1
+/* item option: the drive amplitudes come from control_params (as the environment's
+ * set_action wrote them, phases included) instead of the item's pa */
+#define CBC_SERVE_USE_CONTROL 2
 
 typedef struct cbc_serve cbc_serve;
 
@@ -361,6 +364,8 @@ typedef struct cbc_env_spec {
     float w_distance, w_intensity;                          /* reward weights */
     float positive_terminal_reward, negative_terminal_reward;
     double next_time_step;       /* time_step after each step (1e-6); <= 0 keeps the last accepted step */
+    int32_t env_begin;           /* cbc_env_set_action / cbc_env_post act on envs [env_begin, env_begin + */
+    int32_t env_count;           /* env_count) only (0: every env); the buffers keep all NE rows */
 } cbc_env_spec;
 
 typedef struct cbc_env_buffers {

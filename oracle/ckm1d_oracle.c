@@ -38,6 +38,22 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the OpenMP loops over systems (the CPU baseline times 1 and all
+ * cores); returns the count in effect (1 without OpenMP). */
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 #define OR_MAXN 32
 #define OR_MAXK 8

@@ -62,6 +62,8 @@ def lib():
         h.or_fac_ok.argtypes = [_f64]
         h.or_fac_bad.restype = _f64
         h.or_fac_bad.argtypes = [_f64]
+        h.or_set_threads.restype = _i32
+        h.or_set_threads.argtypes = [_i32]
         h.or_linear_trace.restype = _i32
         h.or_linear_trace.argtypes = [_f64, _f64, _f64, _f64, ctypes.POINTER(_Tol), _i32, _pd, _pd, _pi]
         _lib = h
@@ -118,9 +120,10 @@ def _c64(a):
     return a
 
 
-def set_threads(n: Optional[int]):
-    if n is not None:
-        os.environ["OMP_NUM_THREADS"] = str(n)
+def set_threads(n: Optional[int]) -> int:
+    """OpenMP threads of the oracle's loops over systems (n <= 0 or None: leave as is);
+    returns the count in effect."""
+    return int(lib().or_set_threads(int(n or 0)))
 
 
 def solve(spec: OracleSpec, bufs: dict, kernel_steps: int = 2048, max_steps: int = 10 ** 9) -> dict:

@@ -237,12 +237,13 @@ def test_scene_positions_kernel_matches_oracle(device):
 # ---------------------------------------------------------------------------
 # PosNBC1D against the oracle environment
 
-def _oracle_env(env):
+def _oracle_env(env, num_envs=None):
     from oracle import env_step as E
 
     parts = [E.Part(c.name, c.min, c.max, c.stack) for c in env.observation_space_schema.components]
     rp = env.reward_params
-    return E.EnvOracle(env.num_envs, env.num_bubbles, parts, action_rows=env._action_map._dev_cols,
+    ne = env.num_envs if num_envs is None else num_envs
+    return E.EnvOracle(ne, env.num_bubbles, parts, action_rows=env._action_map._dev_cols,
                        action_scales=env._action_map._scales, pa_cols=list(env._action_map._feat_col["PA"]),
                        position_tolerance=env.position_tolerance, dnorm=1.0 / rp["r_dnorm"],
                        reward_exp=rp["reward_exp"], weights=rp["weights"].cpu().tolist(),

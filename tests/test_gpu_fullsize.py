@@ -10,7 +10,7 @@ test_gpu_plain.py (final states 1e-6 relative, flags equal, attempt counts 2 %).
 import numpy as np
 import pytest
 
-from test_gpu_plain import ATOL_FLOOR, FLOOR_FRAC, RTOL
+from test_gpu_plain import ATOL_FLOOR, FLOOR_FRAC, RTOL, check_counts
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ def _device_rows(m):
     d = m.runtime.device_buffers
     return {"state": d.state.actual_state.cpu().numpy(), "t": d.state.actual_time.cpu().numpy(),
             "h": d.state.time_step.cpu().numpy(), "flags": d.status.status_flags.cpu().numpy(),
-            "steps": d.status.total_steps.cpu().numpy()}
+            "steps": d.status.total_steps.cpu().numpy(), "rejected": d.status.rejected_steps.cpu().numpy()}
 
 
 def _assert_equal_rows(a, b, what):
@@ -50,9 +50,7 @@ def _oracle_sample(w, idx, rows, n):
              + ATOL_FLOOR)
     err = np.abs(got - want) / scale
     assert err.max() <= RTOL, f"max rel err {err.max():.3e}"
-    steps = rows["steps"][idx].astype(np.int64)
-    rel = np.abs(steps - cnt["total_steps"]) / np.maximum(cnt["total_steps"], 1)
-    assert rel.max() <= 0.02
+    check_counts(rows["steps"][idx], rows["rejected"][idx], cnt, failed=ref["status_flags"] < 0)
     return float(err.max())
 
 
