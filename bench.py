@@ -24,6 +24,8 @@ bubbles with implicit radial/translational coupling):
   value = integrated periods of the timed requests / the span from their first
   arrival to their last completion, continuous path under the Poisson load;
   latency = request arrival to completion, p50/p99 over the timed requests.
+* ``plain`` -- the CPU-runnable case: 256 systems of N=2, one 5-period RL step at
+  rtol = atol = 1e-9; a step is one solve of the ensemble from its initial state.
 * ``param_study`` -- 65 536 systems of N=2 on a 256 x 256 (PA1, PA2) grid over
   [0, 2] bar, R0 = (44, 60) um, 200 periods with per-period max R/R0 and mean x
   for periods 51..200, rtol = atol = 1e-10; a step is ``--periods-per-step``
@@ -72,7 +74,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "coupled-bubble-control_amd"))
 sys.path.insert(0, str(ROOT))
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
+from coupledbubble_control.profiling import PEAK_FP64_TFLOPS, flops_per_attempt  # noqa: E402
+
 # HBM bytes per system of one launch of the parameter-study kernel over the whole 200-period
 # job, measured with rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a copy
 # kernel of known size): 539 B read + 3 416 B written, the writes mostly the per-period window
@@ -83,28 +86,17 @@ PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
 # rocprofv3 over a default serving run (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of
 # a copy kernel of known size), total bytes of all their dispatches / env steps served;
 # ring, control words and outputs live in mapped host memory, so this bounds the HBM bytes
-# from above.  None until measured for the current kernels.
-SERVING_TRAFFIC_BYTES_PER_ENV_STEP = None
+# from above: 2.08 GB read + 0.29 GB written over 475 136 env steps (the warm-up requests and
+# the 28 rounds of `--steps 20 --warmup 5 --serving-plain 0`), i.e. 4 368 + 607 bytes
+# (profiles/serving_16384_N2x12288_N4x4096.md).
+SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 4975
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
-
-
-def flops_per_attempt(n: int) -> int:
-    """Algorithmic FLOPs of one Cash--Karp attempt of an N-bubble system (NK=2):
-    6 RHS evaluations, each 129 N (uncoupled, field) + 49 N(N-1) (explicit
-    coupling and matrix assembly per ordered pair) + the 2N x 2N LU solve
-    (2/3 m^3 + 3/2 m^2 - 7/6 m, m = 2N); plus 228 N for stage sums, the 5th-order
-    solution and the error norm, and 17 for stage times and the controller.
-    Each + - x / and each sin/cos/pow counts as one."""
-    m = 2 * n
-    solve = (2 * m ** 3) / 3 + 1.5 * m ** 2 - 7 * m / 6
-    rhs = 129 * n + 49 * n * (n - 1) + solve
-    return int(round(6 * rhs + 228 * n + 17))
 
 
 def parse():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["serving", "param_study", "mixed", "large", "env", "rollout"],
+    p.add_argument("--workload", choices=["serving", "plain", "param_study", "mixed", "large", "env", "rollout"],
                    default="serving")
     p.add_argument("--steps", type=int, default=None,
                    help="serving: timed rounds of one request per vector env (default 34: 1 122 requests, "
@@ -308,8 +300,7 @@ def bench_serving(args):
         log("cpu baseline (one core)")
         cpu1 = cpu_step_sample(W, ws, args.cpu_seconds * 0.6, "the resident set, one 5-period step each",
                                threads=1)
-        cpu["one_core"] = {k: cpu1[k] for k in ("value", "cores", "sample")}
-        cpu["host"] = host_info()
+        cpu = with_one_core(cpu, cpu1)
     keys = ("wall_s", "requests", "env_steps", "periods", "periods_per_s", "env_steps_per_s", "p50_ms", "p99_ms",
             "mean_ms", "max_ms", "failed_env_steps")
     att = cont["attempts"]
@@ -355,6 +346,91 @@ def bench_serving(args):
 
 
 # ---------------------------------------------------------------------------
+def bench_plain(args):
+    """The CPU-runnable plain workload: 256 systems of N = 2 (R0 ~ U[20, 80] um, PA ~ U[0, 1.5]
+    bar, 25 + 50 kHz standing wave), one RL step of 5 periods at rtol = atol = 1e-9; a step
+    is one solve of the whole ensemble from its initial state (model.solve, kernel_steps 2048)."""
+    import numpy as np
+    import torch
+
+    from coupledbubble_control import workloads as W
+
+    steps = args.steps or 20
+    w = W.plain_cpu(seed=0)
+    m = W.build_model(w)
+    m._ensure_buffers()
+    db = m.runtime.device_buffers
+
+    def one():
+        db.status.actual_event.fill_(-1.0)
+        m.solve(max_steps=10 ** 7, kernel_steps=2048, sync_from_host=True)
+
+    for _ in range(args.warmup):
+        one()
+    torch.cuda.synchronize()
+    db.status.total_steps.zero_()
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    wall = float(np.sum(times))
+    att = int(db.status.total_steps.to(torch.int64).sum().item())
+    periods = float(w.ns * 5.0 * steps)
+    cpu = None
+    if args.cpu_baseline:
+        cpu = with_one_core(cpu_step_sample(W, [w], args.cpu_seconds, "the 256-system plain workload"),
+                            cpu_step_sample(W, [w], args.cpu_seconds * 0.5, "the 256-system plain workload",
+                                            threads=1))
+    F = flops_per_attempt(2)
+    return {
+        "value": periods / wall, "unit": "periods/s", "steps": steps, "ms_per_step": wall * 1e3 / steps,
+        "config": {"workload": "plain_cpu_256xN2: 256 systems x N=2, 5 periods, rtol=atol=1e-9", "systems": w.ns,
+                   "mapping": m.runtime.kernel.info().mapping},
+        "roofline": roofline(att * F, wall, window="timed solves (host launch loop included)",
+                             flops_per_attempt=F, attempts=att),
+        "cpu_baseline": cpu,
+    }
+
+
+def with_one_core(cpu: dict, one: dict) -> dict:
+    """The all-core baseline with its one-core row and the host it ran on."""
+    cpu["one_core"] = {k: one[k] for k in ("value", "cores", "sample")}
+    cpu["host"] = host_info()
+    return cpu
+
+
+def _param_cpu_sample(w, ns: int, target_s: float, threads: int = 0) -> dict:
+    """The C oracle's window integration on growing subsets of the parameter-study grid
+    (20 periods each) until one run takes at least target_s / 3."""
+    import numpy as np
+
+    from coupledbubble_control import workloads as W
+    from oracle import ckm1d_oracle as O
+
+    nthreads = O.set_threads(threads if threads > 0 else cpu_threads())
+    cpu_systems, cpu_periods = (2048 if nthreads > 1 else 128), 20.0
+    for _ in range(6):
+        sub = w.subset(np.linspace(0, ns - 1, cpu_systems).astype(int))
+        sub.t_end[:] = cpu_periods
+        ms = W.build_model(sub)
+        bufs = O.host_buffers_dict(ms)
+        spec = O.spec_from_model(ms)
+        log(f"cpu baseline: {cpu_systems} systems x {cpu_periods:g} periods on {nthreads} threads")
+        c0 = time.perf_counter()
+        O.solve_periodic(spec, bufs, w.period, 0, 0)
+        cwall = time.perf_counter() - c0
+        if cwall >= target_s / 3 or cpu_systems >= ns:
+            break
+        cpu_systems = min(ns, int(cpu_systems * min(8.0, target_s / max(cwall, 1e-3))))
+    return {"value": float(np.sum(bufs["actual_time"]) / cwall), "unit": "periods/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{cpu_systems} grid points x {cpu_periods:g} periods (same physics, rtol=atol=1e-10), "
+                      f"C oracle {'with OpenMP' if nthreads > 1 else 'on one core'}, {cwall:.1f} s"}
+
+
 def bench_param_study(args):
     import numpy as np
     import torch
@@ -414,27 +490,9 @@ def bench_param_study(args):
 
     cpu = None
     if args.cpu_baseline:
-        from oracle import ckm1d_oracle as O
-
-        nthreads = cpu_threads()
-        cpu_systems, cpu_periods = 2048, 20.0
-        for _ in range(4):
-            sub = w.subset(np.linspace(0, ns - 1, cpu_systems).astype(int))
-            sub.t_end[:] = cpu_periods
-            ms = W.build_model(sub)
-            bufs = O.host_buffers_dict(ms)
-            spec = O.spec_from_model(ms)
-            log(f"cpu baseline: {cpu_systems} systems x {cpu_periods:g} periods on {nthreads} threads")
-            c0 = time.perf_counter()
-            O.solve_periodic(spec, bufs, w.period, 0, 0)
-            cwall = time.perf_counter() - c0
-            if cwall >= args.cpu_seconds / 3 or cpu_systems >= ns:
-                break
-            cpu_systems = min(ns, int(cpu_systems * min(8.0, args.cpu_seconds / max(cwall, 1e-3))))
-        cpu = {"value": float(np.sum(bufs["actual_time"]) / cwall), "unit": "periods/s", "cores": nthreads,
-               "kind": "port",
-               "sample": f"{cpu_systems} grid points x {cpu_periods:g} periods (same physics, rtol=atol=1e-10), "
-                         f"C oracle with OpenMP, {cwall:.1f} s"}
+        cpu = _param_cpu_sample(w, ns, args.cpu_seconds)
+        cpu1 = _param_cpu_sample(w, ns, args.cpu_seconds * 0.5, threads=1)
+        cpu = with_one_core(cpu, cpu1)
     return {
         "value": advanced / wall,
         "unit": "periods/s",
@@ -528,7 +586,8 @@ def bench_ensemble(args, large: bool):
     cpu = None
     if args.cpu_baseline:
         log("cpu baseline")
-        cpu = cpu_step_sample(W, ws, args.cpu_seconds, name)
+        cpu = with_one_core(cpu_step_sample(W, ws, args.cpu_seconds, name),
+                            cpu_step_sample(W, ws, args.cpu_seconds * 0.5, name, threads=1))
     return {
         "value": periods / wall,
         "unit": "periods/s",
@@ -548,7 +607,7 @@ def bench_ensemble(args, large: bool):
     }
 
 
-def _env_oracle_sample(n: int, T: float, target_s: float) -> dict:
+def _env_oracle_sample(n: int, T: float, target_s: float, threads: int = 0) -> dict:
     """The oracle environment (oracle/env_step.py) with the C oracle solver on a
     growing number of envs of the env workload, random actions, until one run of
     three steps takes at least target_s / 4."""
@@ -557,6 +616,8 @@ def _env_oracle_sample(n: int, T: float, target_s: float) -> dict:
     from coupledbubble_control.models import CKM1D
     from oracle import ckm1d_oracle as C
     from oracle import env_step as E
+
+    nthreads = C.set_threads(threads if threads > 0 else cpu_threads())
 
     k = 16
     rng = np.random.default_rng(1)
@@ -588,9 +649,9 @@ def _env_oracle_sample(n: int, T: float, target_s: float) -> dict:
         if wall >= target_s / 4 or k >= 1 << 14:
             break
         k = min(1 << 14, int(k * min(8.0, max(2.0, target_s / max(wall, 1e-3)))))
-    return {"value": k * T * 3 / wall, "unit": "periods/s", "cores": cpu_threads(), "kind": "port",
-            "sample": f"3 env steps of {k} envs x {n} bubbles (oracle environment, C oracle solver with "
-                      f"OpenMP), {wall:.1f} s"}
+    return {"value": k * T * 3 / wall, "unit": "periods/s", "cores": nthreads, "kind": "port",
+            "sample": f"3 env steps of {k} envs x {n} bubbles (oracle environment, C oracle solver "
+                      f"{'with OpenMP' if nthreads > 1 else 'on one core'}), {wall:.1f} s"}
 
 
 def bench_env(args):
@@ -741,7 +802,8 @@ def bench_env(args):
         shutil.rmtree(store, ignore_errors=True)
     if args.cpu_baseline:
         log("env: CPU baseline (oracle environment)")
-        out["cpu_baseline"] = _env_oracle_sample(n, T, args.cpu_seconds)
+        out["cpu_baseline"] = with_one_core(_env_oracle_sample(n, T, args.cpu_seconds),
+                                            _env_oracle_sample(n, T, args.cpu_seconds * 0.5, threads=1))
     return out
 
 
@@ -828,7 +890,8 @@ def bench_rollout(args):
     cpu = None
     if args.cpu_baseline:
         log("env: CPU baseline (oracle environment)")
-        cpu = _env_oracle_sample(n, T, args.cpu_seconds)
+        cpu = with_one_core(_env_oracle_sample(n, T, args.cpu_seconds),
+                            _env_oracle_sample(n, T, args.cpu_seconds * 0.5, threads=1))
     F = flops_per_attempt(n)
     best = min((sync, "synchronous"), (grp, f"served_{G}_blocks"), (one, "served_1_block"),
                key=lambda x: x[0]["wall_s"])
@@ -869,6 +932,8 @@ def main():
         out = bench_serving(args)
     elif args.workload == "param_study":
         out = bench_param_study(args)
+    elif args.workload == "plain":
+        out = bench_plain(args)
     elif args.workload == "env":
         out = bench_env(args)
     elif args.workload == "rollout":

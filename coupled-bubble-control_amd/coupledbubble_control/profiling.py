@@ -13,6 +13,14 @@
 * ``param``    -- ``--steps`` launches of the parameter-study kernel
                   (65 536 systems of N=2, 10 periods each), the dominant
                   kernel of that workload, for counter collection.
+* ``sweep``    -- the reference's kernel benchmark sweep
+                  (scripts/benchmarks/prof_simu_time.sh: NS = 256 .. 32 768,
+                  seeds 11 42 55 77 89 119, kernel_steps 512, one warm-up and 5
+                  measured solves of 5 periods from the initial state) on its
+                  scenes (``--scenes``, default 2B_TYPICAL): per NS the mean
+                  solve time, sim-time per wall-second (kernel_prof.py:246-252),
+                  ensemble periods/s and the FP64 roofline fraction; one JSON
+                  object per NS.
 """
 from __future__ import annotations
 
@@ -22,6 +30,22 @@ import sys
 from typing import Dict, Iterable, List, Sequence, Tuple
 
 import numpy as np
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) peak
+
+
+def flops_per_attempt(n: int) -> int:
+    """Algorithmic FLOPs of one Cash--Karp attempt of an N-bubble system (NK=2):
+    6 RHS evaluations, each 129 N (uncoupled, field) + 49 N(N-1) (explicit
+    coupling and matrix assembly per ordered pair) + the 2N x 2N LU solve
+    (2/3 m^3 + 3/2 m^2 - 7/6 m, m = 2N); plus 228 N for stage sums, the 5th-order
+    solution and the error norm, and 17 for stage times and the controller.
+    Each + - x / and each sin/cos/pow counts as one."""
+    m = 2 * n
+    solve = (2 * m ** 3) / 3 + 1.5 * m ** 2 - 7 * m / 6
+    rhs = 129 * n + 49 * n * (n - 1) + solve
+    return int(round(6 * rhs + 228 * n + 17))
+
 
 LATENCY_CASES: Tuple[Tuple[int, str, int], ...] = (
     (2, "lane", 4096), (2, "group", 4096), (3, "group", 4096), (4, "group", 4096), (4, "lane", 4096),
@@ -117,9 +141,59 @@ def param_steps(steps: int = 2, periods_per_step: float = 10.0) -> int:
     return int(db.status.total_steps.to(torch.int64).sum().item())
 
 
+SWEEP_NS = (256, 512, 1024, 2048, 4096, 8192, 16384, 32768)
+SWEEP_SEEDS = (11, 42, 55, 77, 89, 119)
+
+
+def sweep(scene: str = "2B_TYPICAL", ns_list: Sequence[int] = SWEEP_NS, seeds: Sequence[int] = SWEEP_SEEDS,
+          kernel_steps: int = 512, measured: int = 5, warmup: int = 1) -> List[Dict]:
+    """The reference's NS sweep on one scene; returns one record per NS."""
+    import time
+
+    import torch
+
+    from . import workloads as W
+
+    out = []
+    for ns in ns_list:
+        times, att, periods = [], 0, 0.0
+        for seed in seeds:
+            w = W.harness_scene(scene, ns, seed)
+            m = W.build_model(w)
+            m._ensure_buffers()
+            db = m.runtime.device_buffers
+
+            def one():
+                db.status.actual_event.fill_(-1.0)
+                db.status.total_steps.zero_()
+                m.solve(max_steps=10 ** 7, kernel_steps=kernel_steps, sync_from_host=True)
+            for _ in range(warmup):
+                one()
+            for _ in range(measured):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                one()
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+                att += int(db.status.total_steps.to(torch.int64).sum().item())
+                periods += float(db.state.actual_time.sum().item())
+        t = float(np.mean(times))
+        F = flops_per_attempt(w.n)
+        rec = {"scene": scene, "ns": ns, "bubbles": w.n, "kernel_steps": kernel_steps, "seeds": len(seeds),
+               "mean_solve_s": t, "std_solve_s": float(np.std(times)), "sim_time_per_wall_s": 5.0 / t,
+               "periods_per_s": periods / float(np.sum(times)), "attempts_per_solve": att / len(times),
+               "tflops": att * F / float(np.sum(times)) / 1e12,
+               "frac": att * F / float(np.sum(times)) / 1e12 / PEAK_FP64_TFLOPS,
+               "mapping": m.runtime.kernel.info().mapping}
+        print(json.dumps(rec), flush=True)
+        out.append(rec)
+    return out
+
+
 def main(argv: Sequence[str] = None) -> None:
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    p.add_argument("what", choices=["copy", "latency", "param"])
+    p.add_argument("what", choices=["copy", "latency", "param", "sweep"])
+    p.add_argument("--scenes", default="2B_TYPICAL", help="sweep: comma list of harness scenes")
     p.add_argument("--mib", type=int, default=512)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--cases", default="", help="latency: comma list like 2lane,4group (default: all)")
@@ -132,6 +206,9 @@ def main(argv: Sequence[str] = None) -> None:
         print(f"copy kernel: {copy_kernel(a.mib)} bytes read and written per launch", flush=True)
     elif a.what == "param":
         print(f"parameter study: {param_steps(a.steps)} attempts in {a.steps} launches", flush=True)
+    elif a.what == "sweep":
+        for sc in a.scenes.split(","):
+            sweep(sc)
     else:
         cases = LATENCY_CASES
         if a.cases:

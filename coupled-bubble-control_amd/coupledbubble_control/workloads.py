@@ -171,6 +171,35 @@ def large_coupling(seed: int = 4, ns16: int = 4096, ns2: int = 4096) -> List[Wor
     return [w16, w2]
 
 
+# The benchmark harness's scenes (scripts/benchmarks/kernel_prof.py:65-94): R0 60 um, PA of
+# both harmonics ~ U[0, 1] bar, 25 + 50 kHz, SW_A field, 5 periods; bubble u's position
+# ~ U[lo_u, hi_u] wavelengths.
+HARNESS_SCENES = {
+    "2B_TYPICAL": [(-0.45, -0.05), (0.05, 0.45)],
+    "3B_TYPICAL": [(-0.45, -0.2), (-0.15, 0.15), (0.2, 0.45)],
+    "4B_TYPICAL": [(-0.45, -0.3), (-0.25, -0.025), (0.025, 0.25), (0.3, 0.45)],
+}
+
+
+def harness_scene(scene_id: str = "2B_TYPICAL", ns: int = 4096, seed: int = 11) -> Workload:
+    """A scene of the reference's kernel benchmark with its draw order
+    (build_parameter_ranges, kernel_prof.py:120-150): positions bubble by bubble, then
+    the amplitudes harmonic by harmonic, from numpy default_rng(seed)."""
+    xr = HARNESS_SCENES[scene_id]
+    n = len(xr)
+    rng = np.random.default_rng(seed)
+    R0 = np.full((ns, n), 60.0)
+    X0 = np.empty((ns, n))
+    for u, (lo, hi) in enumerate(xr):
+        X0[:, u] = rng.uniform(lo, hi, size=(ns,))
+    PA = np.empty((2, ns))
+    for k in range(2):
+        PA[k] = rng.uniform(0.0, 1.0, size=(ns,))
+    FR = np.tile(np.array([[25.0], [50.0]]), (1, ns))
+    return Workload(f"harness_{scene_id}_{ns}", n, R0, X0, PA, FR, np.zeros((2, ns)), np.full(ns, 5.0),
+                    field="SW_A")
+
+
 def build_model(w: Workload, backend=None, num_stored_points: int = 0, solver_spec=None, variant=None,
                 block_size: Optional[int] = None, kernel_backend=True):
     """CKM1D model for a workload: parameters built, initial positions and horizons set
