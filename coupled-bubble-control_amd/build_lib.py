@@ -61,7 +61,7 @@ def build(verbose: bool = False) -> Path:
     BUILD.mkdir(parents=True, exist_ok=True)
     LIB.parent.mkdir(parents=True, exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip"))
-    workers = min(len(srcs), max(1, (os.cpu_count() or 4) // 2), 8)
+    workers = min(len(srcs), max(1, os.cpu_count() or 4), 16)
     with ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -1,5 +1,5 @@
 // Dispatch of the group-mapped kernels (cbc_group_kernel.h); the kernels are
-// instantiated per bubble count in cbc_group_n*.hip so they compile in parallel.
+// instantiated per bubble count (2..16) in cbc_group_n*.hip so they compile in parallel.
 #include "cbc_kernels.h"
 
 namespace cbc {
@@ -7,7 +7,17 @@ namespace cbc {
 const void* group_pick_n2(int nk, int field);
 const void* group_pick_n3(int nk, int field);
 const void* group_pick_n4(int nk, int field);
+const void* group_pick_n5(int nk, int field);
+const void* group_pick_n6(int nk, int field);
+const void* group_pick_n7(int nk, int field);
 const void* group_pick_n8(int nk, int field);
+const void* group_pick_n9(int nk, int field);
+const void* group_pick_n10(int nk, int field);
+const void* group_pick_n11(int nk, int field);
+const void* group_pick_n12(int nk, int field);
+const void* group_pick_n13(int nk, int field);
+const void* group_pick_n14(int nk, int field);
+const void* group_pick_n15(int nk, int field);
 const void* group_pick_n16(int nk, int field);
 
 static int pow2_up(int v)
@@ -23,7 +33,17 @@ static const void* group_pick_any(int ups, int nk, int field)
     case 2: return group_pick_n2(nk, field);
     case 3: return group_pick_n3(nk, field);
     case 4: return group_pick_n4(nk, field);
+    case 5: return group_pick_n5(nk, field);
+    case 6: return group_pick_n6(nk, field);
+    case 7: return group_pick_n7(nk, field);
     case 8: return group_pick_n8(nk, field);
+    case 9: return group_pick_n9(nk, field);
+    case 10: return group_pick_n10(nk, field);
+    case 11: return group_pick_n11(nk, field);
+    case 12: return group_pick_n12(nk, field);
+    case 13: return group_pick_n13(nk, field);
+    case 14: return group_pick_n14(nk, field);
+    case 15: return group_pick_n15(nk, field);
     case 16: return group_pick_n16(nk, field);
     default: return nullptr;
     }

@@ -1,0 +1,6 @@
+// Group-mapped kernels for N = 13 bubbles per system (all harmonic counts and field types).
+#include "cbc_group_kernel.h"
+
+namespace cbc {
+const void* group_pick_n13(int nk, int field) { return group_pick<13>(nk, field); }
+}  // namespace cbc

@@ -1,0 +1,6 @@
+// Group-mapped kernels for N = 7 bubbles per system (all harmonic counts and field types).
+#include "cbc_group_kernel.h"
+
+namespace cbc {
+const void* group_pick_n7(int nk, int field) { return group_pick<7>(nk, field); }
+}  // namespace cbc

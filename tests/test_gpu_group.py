@@ -18,12 +18,43 @@ def _mixed(n, ns, periods, seed=2):
     return w
 
 
+def _any_n(n, ns, periods, seed=9):
+    """Mixed-workload style systems of any bubble count: R0 log-uniform in [10, 100] um,
+    sorted positions on [0, lambda/2] with gaps >= 3 max(R0), PA ~ U[0, 1.5] bar."""
+    from coupledbubble_control import workloads
+
+    rng = np.random.default_rng(seed + n)
+    lam = workloads.wavelength(workloads.water_25c(), 25.0)
+    R0 = np.exp(rng.uniform(np.log(10.0), np.log(100.0), size=(ns, n)))
+    X0 = workloads.sample_positions(rng, ns, n, 0.0, 0.5, 3.0 * R0.max(axis=1) * 1e-6 / lam)
+    PA = rng.uniform(0.0, 1.5, size=(2, ns))
+    FR = np.tile(np.array([[25.0], [50.0]]), (1, ns))
+    return workloads.Workload(f"any_N{n}", n, R0, X0, PA, FR, np.zeros((2, ns)), np.full(ns, periods))
+
+
 @pytest.mark.parametrize("n,ns,periods", [(2, 128, 2.0), (3, 96, 2.0), (4, 64, 1.5), (8, 32, 1.0)])
 def test_group_mapping_parity(device, n, ns, periods):
     from coupledbubble_control.backends import KernelVariant
 
     print(f"case: group mapping N={n} {ns} systems {periods} periods", flush=True)
     w = _mixed(n, ns, periods)
+    m, ref, cnt = _run_pair(w, variant=KernelVariant.GROUP)
+    info = m.runtime.kernel.info()
+    assert info.mapping == "group" and info.local_bytes == 0
+    assert np.array_equal(m.runtime.host_buffers.status.status_flags, ref["status_flags"])
+    _assert_states(m, ref, n)
+    _assert_counts(m, cnt)
+
+
+@pytest.mark.parametrize("n,ns,periods", [(5, 48, 1.0), (6, 40, 1.0), (7, 32, 1.0), (9, 24, 0.5), (10, 16, 0.5),
+                                          (11, 16, 0.5), (12, 16, 0.5), (13, 12, 0.5), (14, 12, 0.5), (15, 12, 0.5)])
+def test_group_mapping_other_bubble_counts(device, n, ns, periods):
+    """Bubble counts that are not powers of two run in groups of next_pow2(N) lanes with
+    padding lanes in the Gauss--Jordan solve (N = 5..7 in 8 lanes, 9..15 in 16)."""
+    from coupledbubble_control.backends import KernelVariant
+
+    print(f"case: group mapping N={n} {ns} systems {periods} periods", flush=True)
+    w = _any_n(n, ns, periods)
     m, ref, cnt = _run_pair(w, variant=KernelVariant.GROUP)
     info = m.runtime.kernel.info()
     assert info.mapping == "group" and info.local_bytes == 0
@@ -113,7 +144,7 @@ def test_auto_mapping_for_two_bubbles(device):
 
 
 def test_no_solver_kernel_uses_scratch(device):
-    """Every kernel the AUTO mapping selects (lane: N = 1, 2; group: N = 2, 3, 4, 8, 16;
+    """Every kernel the AUTO mapping selects (lane: N = 1, 2; group: N = 2 .. 16;
     one or two harmonics; the three field types) keeps its state in registers and LDS: no
     private (scratch) bytes per lane, with the exact-event block compiled into every
     kernel.  The lane kernels for N = 3, 4 (chosen only on request) spill, as recorded."""
@@ -122,7 +153,7 @@ def test_no_solver_kernel_uses_scratch(device):
 
     print("case: scratch bytes of every solver kernel", flush=True)
     seen = []
-    for variant, ns_list in ((KernelVariant.LANE, (1, 2, 3, 4)), (KernelVariant.GROUP, (2, 3, 4, 8, 16))):
+    for variant, ns_list in ((KernelVariant.LANE, (1, 2, 3, 4)), (KernelVariant.GROUP, tuple(range(2, 17)))):
         for n in ns_list:
             for nk in (1, 2):
                 for field in ("CONST", "SW_N", "SW_A"):
