@@ -43,12 +43,16 @@ struct GroupLds {
     static constexpr int G = GroupShape<N>::G;
     static constexpr int n = 2 * N;
     static constexpr bool R = GroupShape<N>::redundant;
-    double mx[3][N][N];        // mx[c][i][j]: effect of bubble j on bubble i
-    double xh[8][G];           // x, H0..H6 of every bubble
-    double ev[2][G];           // collision radius and position
+    // Most frequently accessed first: ds_read2/ds_write2 address two doubles at
+    // 8-bit offsets in units of 8 bytes (up to 2040 bytes from the base
+    // register), so the per-column pivot rows and the bubble data stay in reach
+    // of the group's base address without an address add per access.
     double piv[2][R ? 1 : n + 1];   // pivot row, double buffered (Gauss--Jordan)
     double sol[R ? 1 : n];          // solved accelerations (Gauss--Jordan)
     double rows[R ? n : 1][n + 1];  // the augmented system (redundant solve)
+    double xh[8][G];           // x, H0..H6 of every bubble
+    double ev[2][G];           // collision radius and position
+    double mx[3][N][N];        // mx[c][i][j]: effect of bubble j on bubble i
 };
 
 // Reductions inside a group of G <= 16 lanes (every lane gets the result),
@@ -59,17 +63,20 @@ struct GroupLds {
 // min, OR and the (value, lowest id) maximum do not depend on the order in
 // which elements meet, so the result is a butterfly's (xor partners through
 // LDS-routed shuffles, as these reductions were first written) bit for bit.  The
-// reductions run in wave-uniform code (all lanes active); a lane whose source
-// were inactive would keep its own value.
+// reductions and broadcasts run in wave-uniform code only (all lanes active).
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
 constexpr int kDppHalfMirror = 0x141;
 constexpr int kDppMirror = 0x140;
 
+// A plain DPP move (no "old" operand tied to the destination, so the source
+// is not copied first: two VALU moves per double saved at every step).  Every
+// source lane of these permutations lies in the lane's own row of 16 and every
+// lane is active where they run, so no lane takes the undefined fallback value.
 template <int Ctl>
 __device__ __forceinline__ int dpp_int(int v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, Ctl, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, Ctl, 0xF, 0xF, false);
 }
 
 template <int Ctl>
