@@ -129,6 +129,22 @@ __device__ __forceinline__ void group_argmax(double& v, int& id)
     id = cand;
 }
 
+// group_argmax that also hands every lane the sign bit `neg` of the winning
+// lane's entry: it rides below the row id in the second pass (2 id + neg, so
+// the lowest id still wins), so |a| and the sign give the pivot entry itself
+// without waiting for its row to arrive through LDS.
+template <int G>
+__device__ __forceinline__ void group_argmax_signed(double& v, int& id, bool& neg)
+{
+    double m = v;
+    group_steps<G>([&](auto c) { m = fmax(m, dpp_double<decltype(c)::value>(m)); });
+    int cand = (v == m || m != m) ? (id << 1 | (neg ? 1 : 0)) : 0x7fffffff;
+    group_steps<G>([&](auto c) { cand = min(cand, dpp_int<decltype(c)::value>(cand)); });
+    v = m;
+    id = cand >> 1;
+    neg = (cand & 1) != 0;
+}
+
 // Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
 // quad): a DPP quad_perm broadcast, no LDS.
 template <int G, int J>
@@ -137,6 +153,18 @@ __device__ __forceinline__ double group_bcast(double v)
     static_assert(G == 2 || G == 4, "quad broadcast serves groups of 2 or 4 lanes");
     constexpr int ctl = G == 4 ? J * 0x55 : (J | J << 2 | (2 + J) << 4 | (2 + J) << 6);
     return dpp_double<ctl>(v);
+}
+
+// Orders the group's LDS stores before the loads that follow them in the other
+// lanes of the group.  A workgroup is one wave here (64 threads) and a wave's LDS
+// instructions execute in issue order, so wavefront-scope fences are the whole
+// synchronisation: unlike the workgroup barrier they emit no lgkmcnt(0) wait,
+// so the loads issue right behind the stores instead of after their completion.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int... J, typename F>
@@ -234,7 +262,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                 L.xh[6][i] = o.H5;
                 L.xh[7][i] = o.H6;
             }
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int c = 0; c < 8; ++c)
 #pragma unroll
@@ -293,7 +321,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                         L.rows[N + i][j] = at[j];
                     }
                 }
-                __syncthreads();
+                wave_sync();
 #pragma unroll
                 for (int r = 0; r < n; ++r)
 #pragma unroll
@@ -328,28 +356,34 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             const double vt = used_t ? -1.0 : fabs(at[k]);
             double v = vr;
             int id = i;
+            bool neg = __builtin_signbit(ar[k]);
             if (vt > vr) {
                 v = vt;
                 id = N + i;
+                neg = __builtin_signbit(at[k]);
             }
-            group_argmax<GroupShape<N>::G>(v, id);
+            group_argmax_signed<GroupShape<N>::G>(v, id, neg);
+            // The pivot entry a_kk is the winner's |a| with its sign, bit for bit,
+            // whenever an unused row wins.  (If every unused row is NaN, v is the
+            // -1 mark of a used row and the solve ends NaN whatever inv is.)  Its
+            // reciprocal, the owner's 1 / a_kk too, runs alongside the row's LDS trip.
+            const double inv = rcp_fast(neg ? -v : v);
             const bool own_r = vu && id == i;
             const bool own_t = vu && id == N + i;
             double* pv = L.piv[k & 1];
             if (own_r) {
 #pragma unroll
-                for (int j = k; j <= n; ++j) pv[j] = ar[j];
+                for (int j = k + 1; j <= n; ++j) pv[j] = ar[j];
                 used_r = true;
                 col_r = k;
             }
             if (own_t) {
 #pragma unroll
-                for (int j = k; j <= n; ++j) pv[j] = at[j];
+                for (int j = k + 1; j <= n; ++j) pv[j] = at[j];
                 used_t = true;
                 col_t = k;
             }
-            __syncthreads();
-            const double inv = rcp_fast(pv[k]);  // the owner's pivot: its own 1 / a_kk too
+            wave_sync();
             inv_r = own_r ? inv : inv_r;
             inv_t = own_t ? inv : inv_t;
             const double fr = own_r ? 0.0 : ar[k] * inv;
@@ -365,7 +399,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             L.sol[col_r] = ar[n] * inv_r;
             L.sol[col_t] = at[n] * inv_t;
         }
-        __syncthreads();
+        wave_sync();
         dy[2] = L.sol[ii];
         dy[3] = L.sol[N + ii];
         }
@@ -384,13 +418,13 @@ __device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, do
             L.ev[0][i] = y[0] * rcol;
             L.ev[1][i] = y[1];
         }
-        __syncthreads();
+        wave_sync();
         double gap = 1e30;
         if (vu && i + 1 < N) {
             const double ri = L.ev[0][i], rj = L.ev[0][i + 1];
             gap = fabs(L.ev[1][i] - L.ev[1][i + 1]) - (ri + rj) * colf;
         }
-        __syncthreads();  // ev is rewritten by the next call
+        wave_sync();  // ev is rewritten by the next call
         return group_min<GroupShape<N>::G>(gap);
     }
 }
