@@ -59,6 +59,11 @@ per attempt -- the count in ``flops_per_attempt`` -- times the attempts the
 device counters recorded, over the kernel time) and a CPU baseline: the C
 oracle (same algorithm, OpenMP over systems) timed on a bounded sample of the
 same workload.
+
+The default serving run then measures plain, param_study, mixed, large and env in the
+same process, each at its own defaults without a CPU baseline (``--also``; about a
+minute more), and adds their value, unit, ms per step, FP64 roofline fraction and
+vs-plain ratio to the line under ``other_workloads``.
 """
 from __future__ import annotations
 
@@ -126,6 +131,10 @@ def parse():
                         "evaluate the critic on the next observations each step, as a PPO rollout does")
     p.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample length")
+    p.add_argument("--also", default=None,
+                   help="serving: comma list of further workloads measured after it in the same run, each "
+                        "at its defaults without a CPU baseline, summarised under \"other_workloads\" "
+                        "(default plain,param_study,mixed,large,env; '' for none)")
     return p.parse_args()
 
 
@@ -918,6 +927,37 @@ def bench_rollout(args):
     }
 
 
+OTHER_DEFAULT = "plain,param_study,mixed,large,env"
+
+
+def other_workloads(args, names) -> dict:
+    """Each named workload at its own defaults (no CPU baseline), summarised: value, unit,
+    ms per step, steps, FP64 roofline fraction and the vs-plain ratio where it has one."""
+    import gc
+
+    import torch
+
+    runners = {"plain": bench_plain, "param_study": bench_param_study, "env": bench_env,
+               "rollout": bench_rollout, "mixed": lambda a: bench_ensemble(a, large=False),
+               "large": lambda a: bench_ensemble(a, large=True)}
+    out = {}
+    for name in names:
+        if name not in runners:
+            raise SystemExit(f"--also: unknown workload {name!r}")
+        a = argparse.Namespace(**vars(args))
+        a.workload, a.steps, a.cpu_baseline = name, None, 0
+        gc.collect()
+        torch.cuda.empty_cache()
+        log(f"other workload: {name}")
+        r = runners[name](a)
+        rf = r.get("roofline") or {}
+        out[name] = {"workload": (r.get("config") or {}).get("workload"), "value": r["value"], "unit": r["unit"],
+                     "ms_per_step": r["ms_per_step"], "steps": r["steps"], "warmup": a.warmup,
+                     "tflops": rf.get("achieved"), "frac": rf.get("frac"),
+                     **({"vs_plain": r["vs_plain"]} if r.get("vs_plain") is not None else {})}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -945,6 +985,10 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic"}
     line.update(out)
+    if args.workload == "serving":
+        names = [n for n in (OTHER_DEFAULT if args.also is None else args.also).split(",") if n]
+        if names:
+            line["other_workloads"] = other_workloads(args, names)
     print(json.dumps(line), flush=True)
 
 
