@@ -35,18 +35,22 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",
 ]
-# Group-mapping kernels only: -disable-machine-licm keeps the backend from
+# Group-mapping kernels for N >= 3: -disable-machine-licm keeps the backend from
 # hoisting the step's FP64 constants out of the step loop into SGPR pairs that
-# then spill to VGPR lanes (3-5 % faster per attempt for N = 2..4; the lane
-# kernel measured no gain).
+# then spill to VGPR lanes (3-5 % faster per attempt for N = 3, 4; the lane
+# kernel measured no gain).  The N = 2 pair kernel, since its DPP exchanges no
+# longer copy their operands, runs 2 % faster with the hoisting (9.67 -> 9.48 us
+# per attempt; N = 4 19.0 -> 20.0 us and N = 8 unchanged with it).
 # The environment kernels round every float32 operation separately, as the
 # elementwise torch kernels of the reference environment do.
 EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"], "cbc_env": ["-ffp-contract=off"]}
+NO_EXTRA = {"cbc_group_n2.hip"}
 
 
 def _compile(src: Path, verbose: bool) -> Path:
     obj = BUILD / (src.stem + ".o")
-    extra = [f for prefix, fl in EXTRA_FLAGS.items() if src.name.startswith(prefix) for f in fl]
+    extra = [f for prefix, fl in EXTRA_FLAGS.items()
+             if src.name.startswith(prefix) and src.name not in NO_EXTRA for f in fl]
     cmd = [HIPCC, *CFLAGS, *extra, "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
