@@ -8,10 +8,10 @@
 // runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
 // DPP argmax across the group; its owner stages it in LDS (double
-// buffered, one barrier per column) and every lane eliminates that column from
-// its rows.  For N <= 4 (n <= 8) the group instead stages its n rows in LDS
-// once and every lane solves the whole system in registers with the lane
-// mapping's LU (identical results on every lane, one barrier instead of n).
+// buffered, one wavefront-scope sync per column) and every lane eliminates that
+// column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
+// in LDS once and every lane solves the whole system in registers with the lane
+// mapping's LU (identical results on every lane, one sync instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
 // coupling constants mx are staged in LDS per system; pairs (N = 2) pass the
 // bubble data and the rows by DPP broadcasts instead.  All lanes of a group
@@ -339,7 +339,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             dy[3] = d3;
         } else {
         // Gauss--Jordan with partial pivoting across the group.  Every solution
-        // slot starts as NaN (the barrier above ordered this after the previous
+        // slot starts as NaN (the sync above ordered this after the previous
         // solve's reads): a column that finds no finite unused row (NaN entries)
         // picks a used row again, so some slot is never written and the
         // accelerations come out NaN, never stale values of the previous solve.
@@ -431,7 +431,7 @@ __device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, do
 
 // Root theta of the gap along the step y0 -> y1 of length h (exact event mode),
 // identical on every lane of the group.  Every lane of the wave runs all
-// iterations (the gap evaluation has barriers); a group that converged keeps
+// iterations (the gap evaluation syncs and reduces across the wave); a group that converged keeps
 // its iterate, so the result is the early-exit one of the lane mapping.
 template <int N>
 __device__ double group_event_root(GroupLds<N>& L, int i, bool vu, double rcol, const double (&y0)[4],

@@ -5,7 +5,7 @@
 // Step logic is that of the reference stepper (rkck45_coupled_numba_stepper.py:179-706)
 // as in cbc_lane.hip; every per-system quantity (t, h, event value, error ratio,
 // flags) is held redundantly by all lanes of the group.  The whole wave runs
-// every attempt together (the RHS has barriers), so lanes of idle groups
+// every attempt together (the RHS syncs and reduces across the wave), so lanes of idle groups
 // compute but never store.
 //
 // Plain mode: group = system blockIdx.x * S + group index.
@@ -173,7 +173,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
     return true;
 }
 
-// One attempt for every group of the wave (uniform: contains barriers).  `busy`
+// One attempt for every group of the wave (uniform: wave-wide syncs).  `busy`
 // marks groups with a live system; returns true for a busy group whose system
 // stopped this attempt.
 template <int N, int NK, int FIELD>
