@@ -931,8 +931,9 @@ OTHER_DEFAULT = "plain,param_study,mixed,large,env"
 
 
 def other_workloads(args, names) -> dict:
-    """Each named workload at its own defaults (no CPU baseline), summarised: value, unit,
-    ms per step, steps, FP64 roofline fraction and the vs-plain ratio where it has one."""
+    """Each named workload at its own defaults (no CPU baseline, at most one warm-up step),
+    summarised: value, unit, ms per step, steps, FP64 roofline fraction and the vs-plain
+    ratio where it has one."""
     import gc
 
     import torch
@@ -945,7 +946,7 @@ def other_workloads(args, names) -> dict:
         if name not in runners:
             raise SystemExit(f"--also: unknown workload {name!r}")
         a = argparse.Namespace(**vars(args))
-        a.workload, a.steps, a.cpu_baseline = name, None, 0
+        a.workload, a.steps, a.cpu_baseline, a.warmup = name, None, 0, min(args.warmup, 1)
         gc.collect()
         torch.cuda.empty_cache()
         log(f"other workload: {name}")
