@@ -91,10 +91,10 @@ PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
 # rocprofv3 over a default serving run (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of
 # a copy kernel of known size), total bytes of all their dispatches / env steps served;
 # ring, control words and outputs live in mapped host memory, so this bounds the HBM bytes
-# from above: 2.08 GB read + 0.29 GB written over 475 136 env steps (the warm-up requests and
-# the 28 rounds of `--steps 20 --warmup 5 --serving-plain 0`), i.e. 4 368 + 607 bytes
+# from above: 2.09 GB read + 0.29 GB written over 475 136 env steps (the warm-up requests and
+# the 28 rounds of `--steps 20 --warmup 5 --serving-plain 0`), i.e. 4 398 + 605 bytes
 # (profiles/serving_16384_N2x12288_N4x4096.md).
-SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 4975
+SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 5004
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
