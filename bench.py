@@ -13,10 +13,16 @@ bubbles with implicit radial/translational coupling):
   steps, ~82 ms of arrivals).  The request stream holds ``--warmup`` rounds,
   then the ``--steps`` timed rounds, then 3 more rounds that keep the load on
   while the timed requests finish; value and latency cover the timed requests
-  only (steady state).  The same requests are then served by the
+  only (steady state).  Steps are ordered per env, the data dependency of a
+  resident system: an env's step of a request starts as soon as its own step of
+  the vector env's previous request has finished (the same requests are also
+  served with each request waiting for the vector env's whole previous step,
+  ``continuous_venv_order``).  The same requests are then served by the
   plain path (one kernel launch per batch, per bubble count), and both paths
   are also driven closed-loop (each vector env resubmits as soon as its step
-  completes) for ``--saturation-s`` seconds to measure capacity.  "Plain" is
+  completes) for ``--saturation-s`` seconds to measure capacity, and the
+  continuous path once more with asynchronous envs (each env resubmits as soon
+  as its own step completes, ``saturation.async_*``).  "Plain" is
   static batching: everything queued when the previous batch has finished is
   one batch (one launch per bubble count), and the next batch starts when all
   of its launches have ended; ``plain_per_bin`` also reports one such batch
@@ -240,23 +246,25 @@ def bench_serving(args):
     lo, hi = args.warmup * V, (args.warmup + steps) * V
     lam = W.wavelength(W.water_25c(), 25.0)
 
-    def engine(mode, per_bin=False):
+    def engine(mode, per_bin=False, order="env"):
         bins_w = W.serving_bins(seed=3, sizes=sizes, periods=K)
         bins = {n: W.build_model(w) for n, w in bins_w.items()}
         eng = ServingEngine(bins, periods=K, mode=mode, reset_prob=0.001,
                             reset_scene=lambda n, k, rng: c0_scene(n, k, rng, lam),
-                            serve_opts=sopts, seed=3, plain_per_bin=per_bin)
+                            serve_opts=sopts, seed=3, plain_per_bin=per_bin, order=order)
         return eng, bins
 
     sopts = ServeOptions(max_attempts=1 << 20, **({"poll_interval": args.poll_interval} if args.poll_interval else {}))
 
-    def run(mode, reqs=None, closed=None, per_bin=False, window=True):
-        eng, bins = engine(mode, per_bin)
+    def run(mode, reqs=None, closed=None, per_bin=False, window=True, order="env", asynchronous=False):
+        eng, bins = engine(mode, per_bin, order)
         a0 = attempts_of(bins.values())
         gc.collect()
         gc.disable()   # no collector pass inside the timed request loop (as timeit does)
         try:
-            if closed is not None:
+            if asynchronous:
+                st = eng.run_async(closed, seed=9, timeout_s=300.0, progress=log)
+            elif closed is not None:
                 first, feeder = closed_loop(venvs, closed, seed=9)
                 st = eng.run(first, timeout_s=300.0, progress=log, feeder=feeder)
             else:
@@ -280,6 +288,10 @@ def bench_serving(args):
     whole, cont, info = run("continuous", reqs())
     log(f"continuous window: {cont['periods_per_s']:.4g} periods/s p50 {cont['p50_ms']:.2f} ms "
         f"p99 {cont['p99_ms']:.2f} ms over {cont['requests']} requests")
+    log("continuous, steps ordered per vector env: same requests")
+    _, cont_v, _ = run("continuous", reqs(), order="venv")
+    log(f"continuous (vector-env order) window: {cont_v['periods_per_s']:.4g} periods/s "
+        f"p50 {cont_v['p50_ms']:.2f} ms p99 {cont_v['p99_ms']:.2f} ms")
     plain = plain_bin = sat = None
     if args.serving_plain:
         log("plain (static batches): same requests")
@@ -294,12 +306,24 @@ def bench_serving(args):
             sc, _, _ = run("continuous", closed=args.saturation_s)
             log(f"closed loop {args.saturation_s:g} s: plain")
             sp, _, _ = run("plain", closed=args.saturation_s)
+            log(f"closed loop {args.saturation_s:g} s: continuous, asynchronous envs")
+            sa, _, _ = run("continuous", closed=args.saturation_s, asynchronous=True)
             sat = {"continuous_periods_per_s": sc["periods_per_s"], "plain_periods_per_s": sp["periods_per_s"],
                    "ratio": sc["periods_per_s"] / sp["periods_per_s"],
                    "continuous_p50_ms": sc["p50_ms"], "continuous_p99_ms": sc["p99_ms"],
                    "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
                    "continuous_tflops": sc["flops"] / sc["wall_s"] / 1e12,
-                   "continuous_frac": sc["flops"] / sc["wall_s"] / 1e12 / PEAK_FP64_TFLOPS}
+                   "continuous_frac": sc["flops"] / sc["wall_s"] / 1e12 / PEAK_FP64_TFLOPS,
+                   "async_periods_per_s": sa["periods_per_s"],
+                   "async_ratio": sa["periods_per_s"] / sp["periods_per_s"],
+                   "async_p50_ms": sa["p50_ms"], "async_p99_ms": sa["p99_ms"],
+                   "async_env_steps": sa["env_steps"],
+                   "async_tflops": sa["flops"] / sa["wall_s"] / 1e12,
+                   "async_frac": sa["flops"] / sa["wall_s"] / 1e12 / PEAK_FP64_TFLOPS,
+                   "note": "closed loop: each vector env submits its next step when its step completes "
+                           "(ratio: continuous / plain); asynchronous: each env submits its next step "
+                           "when its own step completes (async_ratio against the same plain loop, whose "
+                           "static batches hold every queued env either way)"}
 
     cpu = cpu1 = None
     if args.cpu_baseline:
@@ -330,13 +354,17 @@ def bench_serving(args):
                              f"followed by {COOL} rounds that keep the load on",
                    "mapping": {str(n): i.mapping for n, i in info.items()},
                    "vgprs": {str(n): i.num_regs for n, i in info.items()},
-                   "poll_interval": sopts.poll_interval, "idle_us": sopts.idle_us},
+                   "poll_interval": sopts.poll_interval, "idle_us": sopts.idle_us,
+                   "order": "per env: an env's step of a request starts when its own step of the vector "
+                            "env's previous request has finished (continuous_venv_order: when the whole "
+                            "previous request has)"},
         "latency_ms": {"p50": cont["p50_ms"], "p99": cont["p99_ms"],
                        "samples": cont["requests"],
                        **({"plain_p50": plain["p50_ms"], "plain_p99": plain["p99_ms"]} if plain else {})},
         "vs_plain": cont["periods_per_s"] / plain["periods_per_s"] if plain else None,
         "env_steps_per_s": cont["env_steps_per_s"],
         "continuous": {k: cont[k] for k in keys},
+        "continuous_venv_order": {k: cont_v[k] for k in keys},
         "plain": {k: plain[k] for k in keys} if plain else None,
         "plain_per_bin": {k: plain_bin[k] for k in keys} if plain_bin else None,
         "saturation": sat,

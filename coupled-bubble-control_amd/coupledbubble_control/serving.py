@@ -13,22 +13,32 @@ amplitudes for its envs and K acoustic periods to integrate.
 
 * ``Admission`` (host logic only): requests wait in arrival order; a request
   is admitted when the systems in flight stay within the budget (the memory
-  budget divided by the bytes one in-flight system holds), and only after the
-  previous step of the same vector env has finished.  A request that does not
-  fit the budget holds back the ones behind it (no starvation of big bursts).
+  budget divided by the bytes one in-flight system holds).  A request that does
+  not fit the budget holds back the ones behind it (no starvation of big
+  bursts).  With ``order="venv"`` a request is also admitted only after the
+  previous step of the same vector env has finished.
 * ``mode="continuous"``: one persistent serving kernel per bubble count
   (``backends.hip.serve.ServeQueue``); admitted requests are appended to its
-  work ring at once and waves with free lanes pick them up while other
-  requests are still integrating.
+  work ring and waves with free lanes pick them up while other requests are
+  still integrating.  ``order="env"`` (default) orders steps per env, the data
+  dependency of a resident system: an env's item of a request is published as
+  soon as that env's item of the previous request has finished, so the next
+  step of a vector env overlaps the slowest systems of its previous one.
+  ``order="venv"`` publishes a request only when the previous step of its
+  vector env has finished as a whole (a vector env that blocks on its step).
 * ``mode="plain"``: static batching, one launch per batch: when the previous
   batch has finished, every admitted request is written into the device
   buffers and one kernel launch per bubble count integrates them; the batch
   completes when its last launch ends (the reference's blocking ``env.step``
   over everything queued).  ``plain_per_bin=True`` runs one such batch loop per
   bubble count, so the bins' batches overlap each other.
-* After a request completes, its envs that did not end with SUCCESS (as in
-  ``check_termination``, :771-789) and, with probability ``reset_prob`` per
-  step, any of its envs start new episodes with fresh radii and positions.
+* After an env step completes (``order="env"``) or a request completes
+  (``order="venv"`` and the plain path), its envs that did not end with SUCCESS
+  (as in ``check_termination``, :771-789) and, with probability ``reset_prob``
+  per step, any of its envs start new episodes with fresh radii and positions.
+* ``run_async`` (continuous path) drives every env on its own: each env submits
+  its next step as soon as its previous one has finished (asynchronous envs, the
+  capacity of the serving kernels when no env waits for another).
 
 A system's arithmetic does not depend on which other systems run beside it,
 so a request's results are the same alone, in the mix, and in either mode.
@@ -66,6 +76,8 @@ class Request:
     periods: float = 0.0
     seq: Optional[np.ndarray] = None
     attempts: int = -1              # step attempts of its items (continuous mode; -1 unknown)
+    left: int = 0                   # envs whose step has not finished (order="env")
+    tau: Optional[np.ndarray] = None   # tau reached per env (order="env")
 
     @property
     def latency(self) -> float:
@@ -77,12 +89,14 @@ class Request:
 
 
 class Admission:
-    """Arrival-ordered admission under a budget of systems in flight."""
+    """Arrival-ordered admission under a budget of systems in flight; with
+    ``per_venv`` a vector env has at most one request admitted at a time."""
 
-    def __init__(self, max_inflight: int):
+    def __init__(self, max_inflight: int, per_venv: bool = True):
         if max_inflight < 1:
             raise ValueError("max_inflight must be >= 1")
         self.max_inflight = int(max_inflight)
+        self.per_venv = bool(per_venv)
         self.pending: Deque[Request] = deque()
         self.inflight = 0
         self.busy_venvs: set = set()
@@ -101,7 +115,7 @@ class Admission:
         stop = False
         while self.pending:
             r = self.pending.popleft()
-            if stop or r.venv in blocked or (only_bin is not None and r.n != only_bin):
+            if stop or (only_bin is not None and r.n != only_bin) or (self.per_venv and r.venv in blocked):
                 blocked.add(r.venv)
                 keep.append(r)
                 continue
@@ -110,8 +124,9 @@ class Admission:
                 keep.append(r)
                 continue
             room -= r.size
-            blocked.add(r.venv)
-            self.busy_venvs.add(r.venv)
+            if self.per_venv:
+                blocked.add(r.venv)
+                self.busy_venvs.add(r.venv)
             out.append(r)
         self.pending = keep
         self.inflight = self.max_inflight - room
@@ -120,6 +135,10 @@ class Admission:
     def finish(self, r: Request) -> None:
         self.inflight -= r.size
         self.busy_venvs.discard(r.venv)
+
+    def release(self, k: int) -> None:
+        """k env steps of admitted requests have finished (per-env accounting)."""
+        self.inflight -= int(k)
 
 
 def c0_scene(n: int, k: int, rng, lam: float) -> Tuple[np.ndarray, np.ndarray]:
@@ -139,7 +158,7 @@ class ServingEngine:
     def __init__(self, bins: Dict[int, object], periods: float = 5.0, mode: str = "continuous",
                  budget_bytes: int = 64 << 20, serve_opts=None, reset_prob: float = 0.0,
                  reset_scene: Optional[Callable] = None, seed: int = 0, keep_step: bool = True,
-                 plain_kernel_steps: int = 1 << 20, plain_per_bin: bool = False):
+                 plain_kernel_steps: int = 1 << 20, plain_per_bin: bool = False, order: str = "env"):
         import torch
 
         from .backends.hip import rkck45_coupled_hip_solver as hs
@@ -147,6 +166,11 @@ class ServingEngine:
 
         if mode not in ("continuous", "plain"):
             raise ValueError("mode must be 'continuous' or 'plain'")
+        if order not in ("env", "venv"):
+            raise ValueError("order must be 'env' or 'venv'")
+        # static batches integrate each system once per launch: a vector env's next
+        # request always waits for the batch that holds its previous one
+        self.order = order if mode == "continuous" else "venv"
         self.torch = torch
         self.hs = hs
         self.bins = dict(bins)
@@ -159,7 +183,7 @@ class ServingEngine:
         self.plain_kernel_steps = int(plain_kernel_steps)
         self.plain_per_bin = bool(plain_per_bin)
         per_sys = max(inflight_bytes(m._execution_spec.ups, m._execution_spec.tile) for m in self.bins.values())
-        self.admission = Admission(max(1, int(budget_bytes) // per_sys))
+        self.admission = Admission(max(1, int(budget_bytes) // per_sys), per_venv=self.order == "venv")
         self.queues: Dict[int, object] = {}
         self.streams: Dict[int, object] = {}
         self.pending_reset = {n: np.zeros(m._execution_spec.ns, dtype=bool) for n, m in self.bins.items()}
@@ -172,9 +196,12 @@ class ServingEngine:
                 o = serve_opts.get(n) if isinstance(serve_opts, dict) else serve_opts
                 o = o or ServeOptions()
                 # ring slots are freed in submission order, so finished items can wait
-                # behind a long-running one: four slots per system that can be in flight
+                # behind a long-running one: four slots per system that can be in flight,
+                # sixteen when envs step on their own (many short steps of other envs
+                # then pass one long step)
                 cap = 64
-                while cap < 4 * min(self.admission.max_inflight, m._execution_spec.ns):
+                per = 16 if self.order == "env" else 4
+                while cap < per * min(self.admission.max_inflight, m._execution_spec.ns):
                     cap *= 2
                 o = ServeOptions(**{**o.__dict__, "capacity": min(max(o.capacity, cap), 1 << 24)})
                 self.queues[n] = ServeQueue(m, o)
@@ -184,6 +211,21 @@ class ServingEngine:
                 db = m.runtime.device_buffers
                 db.state.time_end.copy_(db.state.actual_time)
         torch.cuda.synchronize()
+        self._init_bookkeeping({n: m._execution_spec.ns for n, m in self.bins.items()})
+
+    def _init_bookkeeping(self, ns_of: Dict[int, int]) -> None:
+        """Host-side state of the request loop for bins of ``ns_of[n]`` systems.  Per-env
+        bookkeeping of order="env": the request (id) and position of the env's current
+        item, its sequence number, whether it is published and unfinished, the env's
+        later items in order (and their count), and reserved items not yet published."""
+        self.env_req = {n: np.full(ns, -1, dtype=np.int64) for n, ns in ns_of.items()}
+        self.env_pos = {n: np.zeros(ns, dtype=np.int64) for n, ns in ns_of.items()}
+        self.env_seq = {n: np.zeros(ns, dtype=np.uint32) for n, ns in ns_of.items()}
+        self.env_live = {n: np.zeros(ns, dtype=bool) for n, ns in ns_of.items()}
+        self.env_wait: Dict[int, Dict[int, Deque[Tuple[Request, int]]]] = {n: {} for n in ns_of}
+        self.wait_count = {n: np.zeros(ns, dtype=np.int64) for n, ns in ns_of.items()}
+        self.ready: Dict[int, List[Tuple[Request, np.ndarray]]] = {n: [] for n in ns_of}
+        self.live: Dict[int, Request] = {}
         self.inflight: List[Request] = []
         self.batches: Dict[int, Tuple[object, List[Request]]] = {}
         self.completed: List[Request] = []
@@ -277,6 +319,197 @@ class ServingEngine:
         self.inflight = keep
         return n_done
 
+    # -- continuous, per-env order ----------------------------------------------------
+    def _reset_envs(self, n: int, ids: np.ndarray, flags: np.ndarray) -> None:
+        """New episodes for finished envs ``ids`` of bin n that failed (flag != SUCCESS)
+        or are drawn with ``reset_prob``: their rows are staged for their next item."""
+        if self.reset_scene is None or ids.size == 0:
+            return
+        mask = flags != 1
+        if self.reset_prob > 0.0:
+            mask |= self.rng.random(ids.size) < self.reset_prob
+        if not mask.any():
+            return
+        ids = ids[mask]
+        R0, X0 = self.reset_scene(n, ids.size, self.rng)
+        state, up, cm = self.bins[n].scene_rows(ids, R0, X0)
+        self.queues[n].stage(ids, state, up, cm)
+        self.pending_reset[n][ids] = True
+        self.resets += int(ids.size)
+
+    def _take(self, n: int, r: Request, pos: np.ndarray) -> None:
+        """Reserve the envs at positions ``pos`` of request r for their next item."""
+        ids = r.env_ids[pos]
+        self.env_req[n][ids] = r.rid
+        self.env_pos[n][ids] = pos
+        self.ready[n].append((r, pos))
+
+    def _admit_env(self, reqs: List[Request], now: float) -> None:
+        """Admitted requests, in arrival order: envs that are idle (no item running and
+        none queued before this one) are reserved for publication, the others queue
+        behind their env's earlier steps."""
+        for r in reqs:
+            n = r.n
+            r.start = now
+            r.left = r.size
+            r.flags = np.zeros(r.size, dtype=np.int32)
+            r.tau = np.zeros(r.size)
+            r.attempts = 0
+            self.live[r.rid] = r
+            ids = np.asarray(r.env_ids)
+            wait = self.env_wait[n]
+            free = (self.env_req[n][ids] < 0) & (self.wait_count[n][ids] == 0)
+            busy = np.flatnonzero(~free)
+            for p in busy.tolist():
+                wait.setdefault(int(ids[p]), deque()).append((r, p))
+            self.wait_count[n][ids[busy]] += 1
+            pos = np.flatnonzero(free)
+            if pos.size:
+                self._take(n, r, pos)
+
+    def _publish(self, now: float) -> None:
+        """Publish every bin's reserved items in one submission (kept for the next round
+        when the ring has too few free slots)."""
+        for n, items in self.ready.items():
+            if not items:
+                continue
+            ids = np.concatenate([np.asarray(r.env_ids)[p] for r, p in items]).astype(np.int32)
+            pa = np.concatenate([r.actions[:, p] for r, p in items], axis=1) * 1.0e5
+            reset = self.pending_reset[n][ids]
+            h0 = np.where(reset, 1e-6, self._h0())
+            seq = self.queues[n].submit(ids, pa, self.periods, h0, reset if reset.any() else None)
+            if seq is None:
+                self.ring_full += 1
+                continue
+            self.pending_reset[n][ids] = False
+            self.env_seq[n][ids] = seq
+            self.env_live[n][ids] = True
+            self.ready[n] = []
+
+    def _collect_env(self, now: float) -> int:
+        """Finished env items: record their results in their requests, start new episodes,
+        reserve each env's next queued item, and complete requests whose envs are all done."""
+        n_done = 0
+        for n, q in self.queues.items():
+            live = np.flatnonzero(self.env_live[n])
+            if live.size == 0:
+                continue
+            fin = live[q.done(live, self.env_seq[n][live])]
+            if fin.size == 0:
+                continue
+            misc = q.out_misc[fin].copy()          # [tau, h, flag, attempts]
+            self.env_live[n][fin] = False
+            rids = self.env_req[n][fin]
+            pos = self.env_pos[n][fin]
+            self.env_req[n][fin] = -1
+            self.admission.release(fin.size)
+            finished = []
+            for rid in np.unique(rids):
+                sel = rids == rid
+                r = self.live[int(rid)]
+                r.flags[pos[sel]] = misc[sel, 2].astype(np.int32)
+                r.tau[pos[sel]] = misc[sel, 0]
+                r.attempts += int(misc[sel, 3].sum())
+                r.left -= int(sel.sum())
+                if r.left == 0:
+                    finished.append(r)
+            self._reset_envs(n, fin, misc[:, 2])
+            wq = fin[self.wait_count[n][fin] > 0]
+            if wq.size:
+                wait = self.env_wait[n]
+                nxt: Dict[int, Tuple[Request, List[int]]] = {}
+                for e in wq.tolist():
+                    dq = wait[e]
+                    r2, p2 = dq.popleft()
+                    if not dq:
+                        del wait[e]
+                    nxt.setdefault(r2.rid, (r2, []))[1].append(p2)
+                self.wait_count[n][wq] -= 1
+                for r2, ps in nxt.values():
+                    self._take(n, r2, np.array(ps, dtype=np.int64))
+            for r in sorted(finished, key=lambda x: x.rid):
+                del self.live[r.rid]
+                r.periods = float(r.tau.sum())
+                r.done = now
+                self.completed.append(r)
+                n_done += 1
+                if self._feeder is not None:
+                    for nr in self._feeder(r, now):
+                        self.admission.push(nr)
+        return n_done
+
+    def run_async(self, duration: float, seed: int = 0, pa_max: float = 1.5, timeout_s: float = 600.0,
+                  progress: Optional[Callable[[str], None]] = None) -> Dict[str, float]:
+        """Asynchronous envs (continuous path): every resident env submits a step with new
+        amplitudes ~ U[0, pa_max] bar at t = 0 and, until ``duration`` seconds have passed,
+        its next step as soon as its previous one has finished; episode resets as in
+        ``run``.  Returns throughput and per-env-step latency (submission to completion)."""
+        if self.mode != "continuous":
+            raise ValueError("run_async drives the continuous path")
+        rng = np.random.default_rng(seed)
+        nk = {n: m._model_spec.nk for n, m in self.bins.items()}
+        start = {n: np.zeros(m._execution_spec.ns) for n, m in self.bins.items()}
+        lat: List[np.ndarray] = []
+        periods, steps, failed = 0.0, 0, 0
+        self._t0 = time.perf_counter()
+        last_report = 0.0
+
+        def submit(n: int, ids: np.ndarray, now: float) -> None:
+            pa = rng.uniform(0.0, pa_max, size=(nk[n], ids.size)) * 1.0e5
+            reset = self.pending_reset[n][ids]
+            h0 = np.where(reset, 1e-6, self._h0())
+            while True:
+                seq = self.queues[n].submit(ids.astype(np.int32), pa, self.periods, h0,
+                                            reset if reset.any() else None)
+                if seq is not None:
+                    break
+                self.ring_full += 1
+                self.queues[n].poll()
+                time.sleep(2e-5)
+            self.pending_reset[n][ids] = False
+            self.env_seq[n][ids] = seq
+            self.env_live[n][ids] = True
+            start[n][ids] = now
+
+        for n, m in self.bins.items():
+            submit(n, np.arange(m._execution_spec.ns), 0.0)
+        while any(v.any() for v in self.env_live.values()):
+            now = self._now()
+            if now > timeout_s:
+                raise TimeoutError(f"asynchronous run exceeded {timeout_s} s")
+            moved = 0
+            for n, q in self.queues.items():
+                live = np.flatnonzero(self.env_live[n])
+                if live.size == 0:
+                    continue
+                fin = live[q.done(live, self.env_seq[n][live])]
+                if fin.size == 0:
+                    continue
+                misc = q.out_misc[fin].copy()
+                self.env_live[n][fin] = False
+                lat.append(now - start[n][fin])
+                periods += float(misc[:, 0].sum())
+                steps += int(fin.size)
+                failed += int((misc[:, 2] != 1).sum())
+                self._reset_envs(n, fin, misc[:, 2])
+                if now < duration:
+                    submit(n, fin, now)
+                moved += fin.size
+            for q in self.queues.values():
+                q.poll()
+            if not moved:
+                time.sleep(2e-5)
+            if progress and now - last_report > 5.0:
+                last_report = now
+                progress(f"serving[async] t={now:.1f}s env steps {steps}")
+        wall = self._now()
+        ms = np.concatenate(lat) * 1e3 if lat else np.zeros(1)
+        return {"mode": "continuous_async", "wall_s": wall, "env_steps": steps, "periods": periods,
+                "periods_per_s": periods / wall, "env_steps_per_s": steps / wall,
+                "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
+                "mean_ms": float(ms.mean()), "max_ms": float(ms.max()), "failed_env_steps": failed,
+                "resets": self.resets, "ring_full_retries": self.ring_full}
+
     # -- plain -------------------------------------------------------------------
     def _launch_batch(self, n: int, reqs: List[Request], now: float) -> None:
         torch = self.torch
@@ -340,7 +573,7 @@ class ServingEngine:
         self._feeder = feeder
         self._t0 = time.perf_counter()
         last_report = 0.0
-        while arrivals or self.admission.pending or self.inflight or self.batches:
+        while arrivals or self.admission.pending or self.inflight or self.batches or self.live:
             now = self._now()
             if now > timeout_s:
                 raise TimeoutError(f"serving run exceeded {timeout_s} s ({len(self.completed)} requests done)")
@@ -348,7 +581,15 @@ class ServingEngine:
             while arrivals and arrivals[0].arrival <= now:
                 self.admission.push(arrivals.popleft())
                 moved += 1
-            if self.mode == "continuous":
+            if self.mode == "continuous" and self.order == "env":
+                done = self._collect_env(now)
+                adm = self.admission.admit()
+                if adm:
+                    self._admit_env(adm, now)
+                self._publish(now)
+                for q in self.queues.values():
+                    q.poll()
+            elif self.mode == "continuous":
                 done = self._collect(now)
                 adm = self.admission.admit()
                 if adm:
@@ -387,6 +628,8 @@ class ServingEngine:
         periods = sum(r.periods for r in self.completed)
         failed = sum(int((r.flags != 1).sum()) for r in self.completed)
         mode = self.mode + ("_per_bin" if self.mode == "plain" and self.plain_per_bin else "")
+        if self.mode == "continuous":
+            mode += "_" + self.order
         out = {"mode": mode, "wall_s": wall, "requests": len(self.completed), "env_steps": env_steps,
                "periods": periods, "periods_per_s": periods / wall, "env_steps_per_s": env_steps / wall,
                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

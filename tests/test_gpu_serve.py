@@ -167,13 +167,13 @@ def test_serve_vs_oracle():
     print(f"  max rel err {err:.2e}", flush=True)
 
 
-def _engine_run(bins_w, reqs, mode):
+def _engine_run(bins_w, reqs, mode, order="env"):
     from coupledbubble_control import workloads
     from coupledbubble_control.backends.hip.serve import ServeOptions
     from coupledbubble_control.serving import ServingEngine
 
     bins = {n: workloads.build_model(w) for n, w in bins_w.items()}
-    eng = ServingEngine(bins, periods=5.0, mode=mode,
+    eng = ServingEngine(bins, periods=5.0, mode=mode, order=order,
                         serve_opts=ServeOptions(max_attempts=200000, idle_us=5000))
     try:
         stats = eng.run(reqs, timeout_s=120.0)
@@ -182,7 +182,7 @@ def _engine_run(bins_w, reqs, mode):
     out = {}
     for r in eng.completed:
         y, h, t, f = _rows(bins[r.n], r.env_ids)
-        out[r.rid] = (y, h, t, f, r.flags.copy())
+        out[r.rid] = (y, h, t, f, r.flags.copy(), np.array([r.periods]))
     return stats, out
 
 
@@ -216,6 +216,36 @@ def test_serving_alone_vs_mix():
         for a, b in zip(mix[pick], alone[pick]):
             np.testing.assert_array_equal(a, b)
         print(f"  request {pick} ({reqs[pick].size} envs, N={reqs[pick].n}) alone == mix", flush=True)
+
+
+def test_env_order_equals_vector_env_order():
+    """Three steps of every vector env queued at once.  Ordered per env (an env's next step
+    starts when its own previous one has finished, so a vector env's steps overlap) and
+    ordered per vector env (a step starts when the whole previous one has finished), every
+    request's flags and integrated periods and the final resident rows are the same bit for
+    bit, and equal to three plain static batches."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.serving import Request, vector_envs
+
+    print("\n[env order vs vector-env order] bins N=2 x 384, N=4 x 192, 3 steps per vector env", flush=True)
+    sizes = {2: 384, 4: 192}
+    bins_w = workloads.serving_bins(seed=17, sizes=sizes)
+    venvs = vector_envs(sizes, np.random.default_rng(5), lo=64, hi=160)
+
+    def requests():
+        rng = np.random.default_rng(23)
+        return [Request(k * len(venvs) + v, n, v, ids, rng.uniform(0.0, 1.5, size=(2, ids.size)), 0.0)
+                for k in range(3) for v, (n, ids) in enumerate(venvs)]
+
+    st_env, by_env = _engine_run(bins_w, requests(), "continuous", order="env")
+    st_venv, by_venv = _engine_run(bins_w, requests(), "continuous", order="venv")
+    _, by_plain = _engine_run(bins_w, requests(), "plain")
+    print(f"  env order: {st_env['wall_s']:.3f} s, vector-env order: {st_venv['wall_s']:.3f} s", flush=True)
+    assert st_env["requests"] == st_venv["requests"] == 3 * len(venvs)
+    for rid in by_env:
+        for a, b, c in zip(by_env[rid], by_venv[rid], by_plain[rid]):
+            np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(a, c)
 
 
 def test_serve_relaunch_after_idle_and_admission():

@@ -113,3 +113,140 @@ def test_serve_abi_rejects_null_handles():
     assert lib.cbc_serve_submit(None, None, None, 0, 1.0, None, None, None) == _abi.CBC_ERR_INVALID
     assert lib.cbc_serve_poll(None, None) == _abi.CBC_ERR_INVALID
     assert lib.cbc_serve_destroy(None) == _abi.CBC_OK
+
+
+# -- the request loop's per-env ordering, driven by a host-side stand-in for the serving queue
+
+
+class _FakeQueue:
+    """Host stand-in for ServeQueue: an item of env e finishes ``dur[e]`` polls after it is
+    published; every submission is logged as (tick, env, first amplitude)."""
+
+    def __init__(self, ns, dur, refuse_at=()):
+        self.dur = dict(dur)
+        self.tick = 0
+        self.seq = 0
+        self.fin_tick = {}
+        self.log = []
+        self.refuse_at = set(refuse_at)
+        self.calls = 0
+        self.done_seq = np.zeros(ns, dtype=np.int64) - 1
+        self.out_misc = np.zeros((ns, 4))
+        self.live = {}
+
+    def submit(self, ids, pa, t_end, h0=0.0, reset=None):
+        self.calls += 1
+        if self.calls in self.refuse_at:
+            return None
+        seq = np.arange(self.seq, self.seq + ids.size, dtype=np.uint32)
+        self.seq += ids.size
+        for e, s, a in zip(ids.tolist(), seq.tolist(), pa[0].tolist()):
+            assert e not in self.live, f"env {e} published while its previous item runs"
+            self.live[e] = s
+            self.fin_tick[s] = self.tick + self.dur.get(e, 1)
+            self.log.append((self.tick, e, a))
+        return seq
+
+    def poll(self):
+        self.tick += 1
+        for e, s in list(self.live.items()):
+            if self.fin_tick[s] <= self.tick:
+                self.out_misc[e] = (5.0, 1e-3, 1.0, 10.0)
+                self.done_seq[e] = s
+                del self.live[e]
+        return len(self.live)
+
+    def done(self, ids, seq):
+        return self.done_seq[np.asarray(ids)] >= np.asarray(seq, dtype=np.int64)
+
+    def results(self, ids):
+        return None, self.out_misc[np.asarray(ids)].copy()
+
+    def stats(self):
+        return (1, self.seq, self.seq)
+
+    def close(self):
+        pass
+
+
+def _host_engine(queue, ns, order, budget=10 ** 6):
+    from types import SimpleNamespace
+
+    from coupledbubble_control.serving import ServingEngine
+
+    class HostEngine(ServingEngine):
+        def __init__(self):
+            spec = SimpleNamespace(ns=ns)
+            self.bins = {2: SimpleNamespace(_execution_spec=spec, _model_spec=SimpleNamespace(nk=2))}
+            self.mode, self.order, self.periods, self.keep_step = "continuous", order, 5.0, True
+            self.reset_prob, self.reset_scene, self.rng = 0.0, None, np.random.default_rng(0)
+            self.admission = Admission(budget, per_venv=order == "venv")
+            self.queues = {2: queue}
+            self.pending_reset = {2: np.zeros(ns, dtype=bool)}
+            self._init_bookkeeping({2: ns})
+
+    return HostEngine()
+
+
+def _stream(steps=3):
+    """Two vector envs of three envs each, ``steps`` requests each, all arriving at t = 0;
+    request k of venv v carries amplitude 10 v + k for every env."""
+    reqs = []
+    for k in range(steps):
+        for v in range(2):
+            ids = np.arange(3 * v, 3 * v + 3, dtype=np.int32)
+            reqs.append(Request(len(reqs), 2, v, ids, np.full((2, 3), 10.0 * v + k), 0.0))
+    return reqs
+
+
+def _per_env_actions(log):
+    out = {}
+    for _, e, a in log:
+        out.setdefault(e, []).append(a / 1e5)
+    return out
+
+
+@pytest.mark.parametrize("order", ["env", "venv"])
+def test_request_loop_orders_steps_per_env(order):
+    # env 1 is slow: 7 polls per step, the others 1
+    q = _FakeQueue(6, {1: 7})
+    eng = _host_engine(q, 6, order)
+    st = eng.run(_stream(), timeout_s=30.0)
+    assert st["requests"] == 6 and st["env_steps"] == 18
+    # every env ran its three steps in request order
+    for e, acts in _per_env_actions(q.log).items():
+        v = e // 3
+        assert acts == [10.0 * v + k for k in range(3)], (e, acts)
+    for r in eng.completed:
+        assert r.flags.tolist() == [1, 1, 1] and r.periods == 15.0
+    slow_done = min(t for t, e, a in q.log if e == 1 and a == 1e5)   # env 1's 2nd step published
+    others_2nd = [t for t, e, a in q.log if e in (0, 2) and a == 1e5]
+    if order == "env":
+        # envs 0 and 2 start step 2 long before the slow env 1 has finished step 1
+        assert max(others_2nd) < slow_done
+        # requests still complete in order within a vector env
+        rids = [r.rid for r in eng.completed if r.venv == 0]
+        assert rids == sorted(rids)
+    else:
+        assert min(others_2nd) >= slow_done
+    assert eng.admission.inflight == 0 and not eng.live
+
+
+def test_request_loop_env_order_budget_and_full_ring():
+    # budget of 4 in-flight env steps: requests of 3 envs are admitted one at a time
+    # (3 + 3 > 4); the ring refuses the 2nd and 3rd submissions, which are published later
+    q = _FakeQueue(6, {1: 3, 4: 5}, refuse_at=(2, 3))
+    eng = _host_engine(q, 6, "env", budget=4)
+    peak = []
+    orig = eng._publish
+
+    def publish(now):
+        peak.append(eng.admission.inflight)
+        orig(now)
+
+    eng._publish = publish
+    st = eng.run(_stream(), timeout_s=30.0)
+    assert st["requests"] == 6 and st["ring_full_retries"] == 2
+    assert max(peak) <= 4
+    for e, acts in _per_env_actions(q.log).items():
+        assert acts == [10.0 * (e // 3) + k for k in range(3)]
