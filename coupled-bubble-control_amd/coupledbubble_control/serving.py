@@ -78,6 +78,7 @@ class Request:
     attempts: int = -1              # step attempts of its items (continuous mode; -1 unknown)
     left: int = 0                   # envs whose step has not finished (order="env")
     tau: Optional[np.ndarray] = None   # tau reached per env (order="env")
+    env_attempts: Optional[np.ndarray] = None   # step attempts per env (order="env")
 
     @property
     def latency(self) -> float:
@@ -354,6 +355,7 @@ class ServingEngine:
             r.left = r.size
             r.flags = np.zeros(r.size, dtype=np.int32)
             r.tau = np.zeros(r.size)
+            r.env_attempts = np.zeros(r.size, dtype=np.int64)
             r.attempts = 0
             self.live[r.rid] = r
             ids = np.asarray(r.env_ids)
@@ -409,6 +411,7 @@ class ServingEngine:
                 r = self.live[int(rid)]
                 r.flags[pos[sel]] = misc[sel, 2].astype(np.int32)
                 r.tau[pos[sel]] = misc[sel, 0]
+                r.env_attempts[pos[sel]] = misc[sel, 3].astype(np.int64)
                 r.attempts += int(misc[sel, 3].sum())
                 r.left -= int(sel.sum())
                 if r.left == 0:

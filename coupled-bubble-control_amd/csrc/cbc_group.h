@@ -167,6 +167,23 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One entry of a Gauss--Jordan pivot row into LDS.  Merged into ds_write2_b64
+// pairs, these stores made the backend copy both data operands into a fresh
+// register pair first (four extra moves per pair); a volatile LDS store stays a
+// single ds_write_b64 from the row's own registers (N = 8: an attempt 45.0 ->
+// 43.6 us, end states bit for bit the same).  At N = 16 the unmerged stores
+// push the kernel into scratch at the register limit, so it keeps the pairs.
+template <int N>
+__device__ __forceinline__ void pivot_store(double* p, double v)
+{
+    if constexpr (N < 16) {
+        typedef __attribute__((address_space(3))) volatile double lds_vdouble;
+        *(lds_vdouble*)p = v;
+    } else {
+        *p = v;
+    }
+}
+
 template <int... J, typename F>
 __device__ __forceinline__ void for_lanes(std::integer_sequence<int, J...>, F f)
 {
@@ -373,13 +390,13 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             double* pv = L.piv[k & 1];
             if (own_r) {
 #pragma unroll
-                for (int j = k + 1; j <= n; ++j) pv[j] = ar[j];
+                for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, ar[j]);
                 used_r = true;
                 col_r = k;
             }
             if (own_t) {
 #pragma unroll
-                for (int j = k + 1; j <= n; ++j) pv[j] = at[j];
+                for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, at[j]);
                 used_t = true;
                 col_t = k;
             }
