@@ -66,8 +66,8 @@ device counters recorded, over the kernel time) and a CPU baseline: the C
 oracle (same algorithm, OpenMP over systems) timed on a bounded sample of the
 same workload.
 
-The default serving run then measures plain, param_study, mixed, large and env in the
-same process, each at its own defaults without a CPU baseline (``--also``; about a
+The default serving run then measures plain, param_study, mixed, large and env, each
+in a child process at its own defaults without a CPU baseline (``--also``; about a
 minute more), and adds their value, unit, ms per step, FP64 roofline fraction and
 vs-plain ratio to the line under ``other_workloads``.
 """
@@ -95,12 +95,12 @@ from coupledbubble_control.profiling import PEAK_FP64_TFLOPS, flops_per_attempt 
 PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
 # Fabric bytes per env step served by the two persistent serving kernels, measured with
 # rocprofv3 over a default serving run (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of
-# a copy kernel of known size), total bytes of all their dispatches / env steps served;
-# ring, control words and outputs live in mapped host memory, so this bounds the HBM bytes
-# from above: 2.09 GB read + 0.29 GB written over 475 136 env steps (the warm-up requests and
-# the 28 rounds of `--steps 20 --warmup 5 --serving-plain 0`), i.e. 4 398 + 605 bytes
+# a copy kernel of known size), total bytes of the serving dispatches of the per-env-ordered
+# continuous run / its env steps; ring, control words and outputs live in mapped host
+# memory, so this bounds the HBM bytes from above: 2.25 GB read + 0.42 GB written over
+# 458 752 env steps (`--steps 20 --warmup 5 --serving-plain 0`), i.e. 4 903 + 909 bytes
 # (profiles/serving_16384_N2x12288_N4x4096.md).
-SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 5004
+SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 5812
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 
@@ -961,29 +961,32 @@ OTHER_DEFAULT = "plain,param_study,mixed,large,env"
 def other_workloads(args, names) -> dict:
     """Each named workload at its own defaults (no CPU baseline, at most one warm-up step),
     summarised: value, unit, ms per step, steps, FP64 roofline fraction and the vs-plain
-    ratio where it has one."""
-    import gc
+    ratio where it has one.  Each runs in a child process of its own, as it would run
+    alone: the HIP runtime maps streams to the device's four hardware queues by how many
+    streams the process created before, and the mixed and large ensembles put their bins
+    on side streams that must land on separate queues (after the serving runs' streams,
+    two bins of the mixed ensemble shared one and its step took 8.3 s instead of 6.2 s)."""
+    import subprocess
 
-    import torch
-
-    runners = {"plain": bench_plain, "param_study": bench_param_study, "env": bench_env,
-               "rollout": bench_rollout, "mixed": lambda a: bench_ensemble(a, large=False),
-               "large": lambda a: bench_ensemble(a, large=True)}
+    known = ("plain", "param_study", "env", "rollout", "mixed", "large")
     out = {}
     for name in names:
-        if name not in runners:
+        if name not in known:
             raise SystemExit(f"--also: unknown workload {name!r}")
-        a = argparse.Namespace(**vars(args))
-        a.workload, a.steps, a.cpu_baseline, a.warmup = name, None, 0, min(args.warmup, 1)
-        gc.collect()
-        torch.cuda.empty_cache()
-        log(f"other workload: {name}")
-        r = runners[name](a)
-        rf = r.get("roofline") or {}
-        out[name] = {"workload": (r.get("config") or {}).get("workload"), "value": r["value"], "unit": r["unit"],
-                     "ms_per_step": r["ms_per_step"], "steps": r["steps"], "warmup": a.warmup,
+        warm = min(args.warmup, 1)
+        log(f"other workload: {name} (child process)")
+        cmd = [sys.executable, str(Path(__file__).resolve()), "--workload", name, "--cpu-baseline", "0",
+               "--warmup", str(warm)]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=900)
+        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        if r.returncode != 0 or not lines:
+            raise SystemExit(f"other workload {name} failed with exit code {r.returncode}")
+        d = json.loads(lines[-1])
+        rf = d.get("roofline") or {}
+        out[name] = {"workload": (d.get("config") or {}).get("workload"), "value": d["value"], "unit": d["unit"],
+                     "ms_per_step": d["ms_per_step"], "steps": d["steps"], "warmup": warm,
                      "tflops": rf.get("achieved"), "frac": rf.get("frac"),
-                     **({"vs_plain": r["vs_plain"]} if r.get("vs_plain") is not None else {})}
+                     **({"vs_plain": d["vs_plain"]} if d.get("vs_plain") is not None else {})}
     return out
 
 
