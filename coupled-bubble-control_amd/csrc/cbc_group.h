@@ -145,6 +145,31 @@ __device__ __forceinline__ void group_argmax_signed(double& v, int& id, bool& ne
     neg = (cand & 1) != 0;
 }
 
+// Pivot key of a Gauss--Jordan candidate: |a| with its 7 low bits replaced by
+// (63 - row) << 1 | sign(a).  The largest key of the group is the largest |a|
+// (ties within 2^-45 relative go to the lower row), the lowest row among equal
+// ones, and it carries the winner's row, so one max over the group replaces the
+// (value, lowest row) argmax.  NaN and infinite entries give NaN keys, which the
+// max ignores; used rows carry -1, whose key decodes to row 63, owned by no lane.
+__device__ __forceinline__ double pivot_key(double a, int row)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(a));
+    const unsigned long long low = ((unsigned long long)(63 - row) << 1) | (__builtin_signbit(a) ? 1ull : 0ull);
+    return __longlong_as_double((long long)((b & ~0x7Full) | low));
+}
+
+__device__ __forceinline__ int pivot_key_row(double key)
+{
+    return 63 - (int)(((unsigned long long)__double_as_longlong(key) & 0x7Full) >> 1);
+}
+
+template <int G>
+__device__ __forceinline__ double group_max(double v)
+{
+    group_steps<G>([&](auto c) { v = fmax(v, dpp_double<decltype(c)::value>(v)); });
+    return v;
+}
+
 // Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
 // quad): a DPP quad_perm broadcast, no LDS.
 template <int G, int J>
@@ -369,38 +394,56 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         double inv_r = 0.0, inv_t = 0.0;
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-            const double vr = used_r ? -1.0 : fabs(ar[k]);
-            const double vt = used_t ? -1.0 : fabs(at[k]);
-            double v = vr;
-            int id = i;
-            bool neg = __builtin_signbit(ar[k]);
-            if (vt > vr) {
-                v = vt;
-                id = N + i;
-                neg = __builtin_signbit(at[k]);
-            }
-            group_argmax_signed<GroupShape<N>::G>(v, id, neg);
-            // The pivot entry a_kk is the winner's |a| with its sign, bit for bit,
-            // whenever an unused row wins.  (If every unused row is NaN, v is the
-            // -1 mark of a used row and the solve ends NaN whatever inv is.)  Its
-            // reciprocal, the owner's 1 / a_kk too, runs alongside the row's LDS trip.
-            const double inv = rcp_fast(neg ? -v : v);
-            const bool own_r = vu && id == i;
-            const bool own_t = vu && id == N + i;
+            bool own_r, own_t;
             double* pv = L.piv[k & 1];
+            double inv, inv_own = 0.0;
+            if constexpr (N < 16) {
+                // one max over pivot keys; the owner's exact 1 / a_kk travels with
+                // its row in slot k of the buffer (N = 8: an attempt 43.7 -> 41.9 us)
+                const double kr = used_r ? -1.0 : pivot_key(ar[k], i);
+                const double kt = used_t ? -1.0 : pivot_key(at[k], N + i);
+                const int row = pivot_key_row(group_max<GroupShape<N>::G>(fmax(kr, kt)));
+                own_r = vu && row == i;
+                own_t = vu && row == N + i;
+                inv_own = rcp_fast(own_t ? at[k] : ar[k]);
+            } else {
+                // N = 16 sits at the register limit, where the key search's live
+                // ranges spill: the two-pass argmax, whose result gives every lane the
+                // pivot entry, so 1 / a_kk runs alongside the row's LDS trip
+                const double vr = used_r ? -1.0 : fabs(ar[k]);
+                const double vt = used_t ? -1.0 : fabs(at[k]);
+                double v = vr;
+                int id = i;
+                bool neg = __builtin_signbit(ar[k]);
+                if (vt > vr) {
+                    v = vt;
+                    id = N + i;
+                    neg = __builtin_signbit(at[k]);
+                }
+                group_argmax_signed<GroupShape<N>::G>(v, id, neg);
+                // The pivot entry a_kk is the winner's |a| with its sign, bit for bit,
+                // whenever an unused row wins.  (If every unused row is NaN, v is the
+                // -1 mark of a used row and the solve ends NaN whatever inv is.)
+                inv = rcp_fast(neg ? -v : v);
+                own_r = vu && id == i;
+                own_t = vu && id == N + i;
+            }
             if (own_r) {
+                if constexpr (N < 16) pivot_store<N>(pv + k, inv_own);
 #pragma unroll
                 for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, ar[j]);
                 used_r = true;
                 col_r = k;
             }
             if (own_t) {
+                if constexpr (N < 16) pivot_store<N>(pv + k, inv_own);
 #pragma unroll
                 for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, at[j]);
                 used_t = true;
                 col_t = k;
             }
             wave_sync();
+            if constexpr (N < 16) inv = pv[k];
             inv_r = own_r ? inv : inv_r;
             inv_t = own_t ? inv : inv_t;
             const double fr = own_r ? 0.0 : ar[k] * inv;
