@@ -7,9 +7,10 @@
 // (translational) of the 2N x (2N+1) augmented system in VGPRs and the group
 // runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
-// DPP argmax across the group; its owner stages it in LDS (double
-// buffered, one wavefront-scope sync per column) and every lane eliminates that
-// column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
+// DPP reduction across the group (for N < 16 one max over keys that pack |a|, the
+// row and the sign, where entries closer than 2^-45 relative count as ties); its
+// owner stages it in LDS (double buffered, one wavefront-scope sync per column) and
+// every lane eliminates that column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
 // in LDS once and every lane solves the whole system in registers with the lane
 // mapping's LU (identical results on every lane, one sync instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
