@@ -41,8 +41,9 @@ bubbles with implicit radial/translational coupling):
   10 periods add the per-launch tails: 3.09e6 against 3.43e6 periods/s).
 * ``mixed`` -- 32 768 systems with N in {2, 3, 4, 8} at 8:4:2:1 and horizons of
   1..50 periods, rtol = atol = 1e-9: the autotuned persistent work-queue
-  ensemble against one plain launch loop per bubble count; a step is the whole
-  ensemble.
+  ensemble (all bins in one fused persistent kernel when the autotuner maps every
+  bin to lane groups; the per-bin launches on side streams are timed beside it)
+  against one plain launch loop per bubble count; a step is the whole ensemble.
 * ``large`` -- 4 096 systems of N=16 (32 x 32 coupling solve per RHS) mixed 1:1
   with 4 096 systems of N=2, 20 periods, rtol = atol = 1e-9; as ``mixed``.
 * ``env`` -- the vector environment itself: PosNBC1D with 16 384 envs of 2
@@ -582,15 +583,15 @@ def bench_ensemble(args, large: bool):
         MixedEnsemble(ms, tuning).solve()
         torch.cuda.synchronize()
 
-    def timed_queued():
+    def timed_queued(fused="auto"):
         ms = models()
-        ens = MixedEnsemble(ms, tuning)
+        ens = MixedEnsemble(ms, tuning, fused=fused)
         ens._prepare()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rounds = ens.solve()
         torch.cuda.synchronize()
-        return time.perf_counter() - t0, ms, rounds
+        return time.perf_counter() - t0, ms, ens.fused_active
 
     def timed_plain():
         ms = models()
@@ -607,13 +608,20 @@ def bench_ensemble(args, large: bool):
     walls, flops, periods = [], 0.0, 0.0
     for k in range(steps):
         log(f"queued ensemble solve {k + 1}/{steps}")
-        wall, ms, rounds = timed_queued()
+        wall, ms, fused = timed_queued()
         walls.append(wall)
         att = attempts_of(ms)
         flops += sum(att[n] * flops_per_attempt(n) for n in att)
         periods += float(sum(m.runtime.device_buffers.state.actual_time.sum().item() for m in ms))
     wall = sum(walls)
-    log(f"queued: {periods / wall:.4g} periods/s")
+    launch = "one fused persistent kernel" if fused else "one persistent launch per bin on side streams"
+    log(f"queued ({launch}): {periods / wall:.4g} periods/s")
+    side = None
+    if fused:
+        log("queued, one persistent launch per bin on side streams")
+        swall, sms, _ = timed_queued("off")
+        side = float(sum(m.runtime.device_buffers.state.actual_time.sum().item() for m in sms)) / swall
+        log(f"per-bin launches: {side:.4g} periods/s")
     log("plain: one launch loop per bubble count")
     pwall, pms = timed_plain()
     pperiods = float(sum(m.runtime.device_buffers.state.actual_time.sum().item() for m in pms))
@@ -637,7 +645,9 @@ def bench_ensemble(args, large: bool):
                    "tuning_trials_s": {str(n): r.trials for n, r in tune.items()}},
         "vs_plain": (periods / wall) / (pperiods / pwall),
         "plain_periods_per_s": pperiods / pwall,
-        "roofline": roofline(flops / steps, wall / steps, window="timed region (concurrent persistent launches)",
+        "launch": launch,
+        "per_bin_launches_periods_per_s": side,
+        "roofline": roofline(flops / steps, wall / steps, window=f"timed region ({launch})",
                              flops_per_attempt={str(w.n): flops_per_attempt(w.n) for w in ws}),
         "cpu_baseline": cpu,
         "status_counts": {str(n): {str(int(a)): int(b) for a, b in zip(*f)} for n, f in flags.items()},

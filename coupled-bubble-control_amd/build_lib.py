@@ -46,7 +46,8 @@ CFLAGS = [
 # The lane kernel schedules with the AMDGPU register-pressure trackers: 377 instead of 395
 # registers and 5-7 % less time per attempt (11.29 -> 10.68 us at 4 096 systems, 13.40 ->
 # 12.50 us at 65 536), bit for bit the same; the group kernels measured no gain from them.
-EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"], "cbc_env": ["-ffp-contract=off"],
+EXTRA_FLAGS = {"cbc_group_n": ["-mllvm", "-disable-machine-licm"], "cbc_fused": ["-mllvm", "-disable-machine-licm"],
+               "cbc_env": ["-ffp-contract=off"],
                "cbc_lane": ["-mllvm", "-amdgpu-use-amdgpu-trackers"]}
 NO_EXTRA = {"cbc_group_n2.hip"}
 # The redundant-LU group kernels (N = 3, 4) at -O2: 2 % less time per attempt than at -O3

@@ -16,7 +16,7 @@ import torch  # noqa: F401  (load torch's HIP runtime before libcbc)
 _PKG_ROOT = Path(__file__).resolve().parents[3]
 LIB_PATH = Path(os.environ.get("CBC_LIB", _PKG_ROOT / "lib" / "libcbc.so"))
 
-CBC_ABI_VERSION = 4
+CBC_ABI_VERSION = 5
 
 CBC_OK = 0
 CBC_ERR_INVALID = -1
@@ -183,6 +183,9 @@ SIGNATURES = {
     "cbc_solve_queued": (_i32, [_vp, ctypes.POINTER(BuffersC), _vp, _i32, _i32, _i32, _i32, _i64, _vp,
                                 ctypes.POINTER(_i32)]),
     "cbc_launch_queued": (_i32, [_vp, ctypes.POINTER(BuffersC), _vp, _i32, _i32, _i32, _i32, _vp]),
+    "cbc_launch_fused": (_i32, [ctypes.POINTER(_vp), ctypes.POINTER(ctypes.POINTER(BuffersC)), ctypes.POINTER(_vp),
+                                ctypes.POINTER(_i32), ctypes.POINTER(_i32), _i32, _i32, _i32, _vp]),
+    "cbc_fused_supported": (_i32, [ctypes.POINTER(_i32), _i32, _i32, _i32]),
     "cbc_plan_active_count": (_i32, [_vp, _vp, ctypes.POINTER(_i32)]),
     "cbc_serve_create": (_i32, [_vp, ctypes.POINTER(BuffersC), ctypes.POINTER(ServeSpecC), _vp, ctypes.POINTER(_vp)]),
     "cbc_serve_get_view": (_i32, [_vp, ctypes.POINTER(ServeViewC)]),

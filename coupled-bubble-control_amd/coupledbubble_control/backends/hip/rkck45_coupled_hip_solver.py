@@ -201,6 +201,39 @@ def launch_queued(kernel, d_buffers, order: torch.Tensor, waves: int = 0, refill
                                      int(waves), int(refill), int(kernel_steps), _abi.stream_handle(stream)))
 
 
+FUSED_MAX_BINS = 4   # bins of one fused launch (cbc_launch_fused)
+
+
+def fused_supported(bubble_counts, nk: int, field: str) -> bool:
+    """Whether one fused persistent kernel covers bins of these (distinct) bubble counts
+    at `nk` harmonics and acoustic field `field` (ModelSpec.ac)."""
+    ups = list(bubble_counts)
+    if not ups or len(ups) > FUSED_MAX_BINS or len(set(ups)) != len(ups) or field not in _abi.FIELD_CODES:
+        return False
+    arr = (ctypes.c_int32 * len(ups))(*ups)
+    return _abi.check(_abi.lib().cbc_fused_supported(arr, len(ups), int(nk), _abi.FIELD_CODES[field])) == 1
+
+
+def launch_fused(kernels, d_buffers, orders, refills, waves: int = 0, kernel_steps: int = 1 << 20,
+                 stream=None) -> None:
+    """The queued launches of several bins (one plan per bubble count, group mapping) as
+    ONE asynchronous persistent kernel: every wave drains the bins' queues in turn,
+    largest bubble count first, and each bin's systems step exactly as in its own
+    launch_queued.  Capturable; the order tensors must stay alive until it has run."""
+    nb = len(kernels)
+    if not (len(d_buffers) == len(orders) == len(refills) == nb):
+        raise ValueError("one buffer set, order and refill per plan")
+    lib = _abi.lib()
+    bufs = [_buffers_c(b) for b in d_buffers]
+    plans = (ctypes.c_void_p * nb)(*[k.handle.value for k in kernels])
+    bptrs = (ctypes.POINTER(_abi.BuffersC) * nb)(*[ctypes.pointer(b) for b in bufs])
+    optrs = (ctypes.c_void_p * nb)(*[_abi.ptr(o) for o in orders])
+    lens = (ctypes.c_int32 * nb)(*[int(o.numel()) for o in orders])
+    rf = (ctypes.c_int32 * nb)(*[int(r) for r in refills])
+    _abi.check(lib.cbc_launch_fused(plans, bptrs, optrs, lens, rf, nb, int(waves), int(kernel_steps),
+                                    _abi.stream_handle(stream)))
+
+
 def active_count(kernel, stream=None) -> int:
     """Systems left ACTIVE by the plan's last launch (synchronises the stream)."""
     lib = _abi.lib()

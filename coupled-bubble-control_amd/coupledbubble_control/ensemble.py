@@ -68,30 +68,53 @@ class TuneRecord:
     trials: Dict[str, float] = field(default_factory=dict)
 
 
+FUSED_MODES = ("auto", "on", "off")
+
+
 class MixedEnsemble:
-    def __init__(self, models: Sequence, tuning: Optional[Dict[int, BinTuning]] = None):
+    """One CKM1D model per bubble count, integrated together.  ``fused``: "on" runs
+    every bin in one fused persistent kernel (all bins in the group mapping, whatever
+    their tuning says; an error if no fused kernel covers the bubble counts), "off"
+    runs one queued launch per bin on side streams, "auto" fuses when a fused kernel
+    covers the bins and every bin's plan uses the group mapping."""
+
+    def __init__(self, models: Sequence, tuning: Optional[Dict[int, BinTuning]] = None, fused: str = "auto"):
         self.models = list(models)
         ns = [m._execution_spec.ups for m in self.models]
         if len(set(ns)) != len(ns):
             raise ValueError("one model per bubble count")
+        if fused not in FUSED_MODES:
+            raise ValueError(f"fused must be one of {FUSED_MODES}, got {fused!r}")
         self.tuning = {m._execution_spec.ups: (tuning or {}).get(m._execution_spec.ups, BinTuning())
                        for m in self.models}
+        self.fused = fused
+        self.fused_active = False
         self._streams: List[torch.cuda.Stream] = []
         self._orders: List[torch.Tensor] = []
         self._graph = None
 
     # -- set-up ---------------------------------------------------------------
+    def _fusable(self) -> bool:
+        ms = self.models[0]._model_spec
+        return (len({(m._model_spec.nk, m._model_spec.ac) for m in self.models}) == 1
+                and hs.fused_supported([m._execution_spec.ups for m in self.models], ms.nk, ms.ac))
+
     def _prepare(self):
+        fuse = self.fused != "off" and self._fusable()
+        if self.fused == "on" and not fuse:
+            raise ValueError("no fused kernel covers bubble counts "
+                             f"{sorted(m._execution_spec.ups for m in self.models)} at these specs")
         for m in self.models:
             m._ensure_buffers()
             t = self.tuning[m._execution_spec.ups]
-            want = KernelVariant(t.variant)
-            block = t.block or None
+            want = KernelVariant("group" if self.fused == "on" else t.variant)
+            block = None if want == KernelVariant.GROUP else (t.block or None)
             if m.runtime.kernel is None or m.runtime.variant != want or m._hip_opts.block_size != block:
                 m.runtime.variant = want
                 m._hip_opts.block_size = block
                 m.runtime.solver = m._make_solver(m.runtime.backend, want)
-        k = min(len(self.models), MAX_SIDE_STREAMS)
+        self.fused_active = fuse and all(m.runtime.kernel.info().mapping == "group" for m in self.models)
+        k = 0 if self.fused_active else min(len(self.models), MAX_SIDE_STREAMS)
         if len(self._streams) != k:
             self._streams = [torch.cuda.Stream() for _ in range(k)]
 
@@ -102,11 +125,21 @@ class MixedEnsemble:
 
     # -- execution ------------------------------------------------------------
     def launch(self, kernel_steps: int = 1 << 20, orders: Optional[List[torch.Tensor]] = None) -> None:
-        """Issue one persistent launch per bin on its own stream, joined back into the
-        current stream (asynchronous; capturable)."""
+        """Issue the bins' persistent launches (asynchronous; capturable): one fused
+        kernel on the current stream, or one launch per bin on side streams joined back
+        into the current stream."""
         self._prepare()
         orders = orders if orders is not None else self.orders()
         main = torch.cuda.current_stream()
+        if self.fused_active:
+            live = [i for i, o in enumerate(orders) if o.numel() > 0]
+            if live:
+                ms = [self.models[i] for i in live]
+                hs.launch_fused([m.runtime.kernel for m in ms], [m.runtime.device_buffers for m in ms],
+                                [orders[i] for i in live],
+                                [self.tuning[m._execution_spec.ups].refill for m in ms],
+                                waves=0, kernel_steps=kernel_steps, stream=main)
+            return
         for st in self._streams:
             st.wait_stream(main)
         ups = [m._execution_spec.ups for m in self.models]

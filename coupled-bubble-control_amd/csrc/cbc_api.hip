@@ -445,6 +445,82 @@ int32_t cbc_launch_queued(cbc_plan* plan, const cbc_buffers* buf, const int32_t*
     return CBC_OK;
 }
 
+// Bubble counts of a fused launch: distinct, 1..kFusedMaxBins of them.
+static int32_t fused_check_list(const int32_t* ups, int32_t n_bins)
+{
+    if (!ups) return fail(CBC_ERR_INVALID, "fused launch: null bubble-count list");
+    if (n_bins < 1 || n_bins > cbc::kFusedMaxBins)
+        return fail(CBC_ERR_INVALID, "fused launch: n_bins = %d not in [1, %d]", n_bins, cbc::kFusedMaxBins);
+    for (int b = 0; b < n_bins; ++b)
+        for (int c = 0; c < b; ++c)
+            if (ups[b] == ups[c]) return fail(CBC_ERR_INVALID, "fused launch: bubble count %d in two bins", ups[b]);
+    return CBC_OK;
+}
+
+int32_t cbc_fused_supported(const int32_t* ups, int32_t n_bins, int32_t nk, int32_t field)
+{
+    int32_t rc = fused_check_list(ups, n_bins);
+    if (rc != CBC_OK) return rc;
+    int u[cbc::kFusedMaxBins], slot[cbc::kFusedMaxBins];
+    for (int b = 0; b < n_bins; ++b) u[b] = ups[b];
+    return cbc::fused_pick(u, n_bins, nk, field, slot) != nullptr ? 1 : 0;
+}
+
+int32_t cbc_launch_fused(cbc_plan* const* plans, const cbc_buffers* const* bufs, const int32_t* const* orders,
+                         const int32_t* order_lens, const int32_t* refills, int32_t n_bins, int32_t waves,
+                         int32_t kernel_steps, void* stream)
+{
+    if (!plans || !bufs || !orders || !order_lens || !refills)
+        return fail(CBC_ERR_INVALID, "cbc_launch_fused: null argument array");
+    if (n_bins < 1 || n_bins > cbc::kFusedMaxBins)
+        return fail(CBC_ERR_INVALID, "cbc_launch_fused: n_bins = %d not in [1, %d]", n_bins, cbc::kFusedMaxBins);
+    if (waves < 0) return fail(CBC_ERR_INVALID, "waves must be >= 0");
+    int32_t ups[cbc::kFusedMaxBins];
+    for (int b = 0; b < n_bins; ++b) {
+        const cbc_plan* p = plans[b];
+        if (!p || !bufs[b]) return fail(CBC_ERR_INVALID, "cbc_launch_fused: null plan or buffers of bin %d", b);
+        if (p->mapping != CBC_MAP_GROUP)
+            return fail(CBC_ERR_UNSUPPORTED, "cbc_launch_fused: bin %d (N = %d) does not use the group mapping", b,
+                        p->exec.ups);
+        if (p->model.nk != plans[0]->model.nk || p->model.field != plans[0]->model.field)
+            return fail(CBC_ERR_INVALID, "cbc_launch_fused: bins differ in harmonic count or field type");
+        if (p->device != plans[0]->device) return fail(CBC_ERR_INVALID, "cbc_launch_fused: plans on different devices");
+        ups[b] = p->exec.ups;
+    }
+    int32_t rc = fused_check_list(ups, n_bins);
+    if (rc != CBC_OK) return rc;
+    int u[cbc::kFusedMaxBins], slot[cbc::kFusedMaxBins];
+    for (int b = 0; b < n_bins; ++b) u[b] = ups[b];
+    const void* fn = cbc::fused_pick(u, n_bins, plans[0]->model.nk, plans[0]->model.field, slot);
+    if (!fn) return fail(CBC_ERR_UNSUPPORTED, "cbc_launch_fused: no fused kernel covers these bubble counts");
+    cbc::FusedArgs f;
+    std::memset(&f, 0, sizeof(f));
+    for (int b = 0; b < n_bins; ++b) {
+        int32_t w = 1;  // the grid is the fused kernel's, not the bin's
+        rc = queued_args(plans[b], bufs[b], orders[b], order_lens[b], w, refills[b], kernel_steps, f.bin[slot[b]]);
+        if (rc != CBC_OK) return rc;
+    }
+    if (waves == 0) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_fail(e, "device query");
+        cbc::KernelAttr at{};
+        e = cbc::fused_attr(fn, &at);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
+        const int regs = at.num_regs > 0 ? at.num_regs : 512;
+        waves = cus * 4 * (512 / regs > 0 ? 512 / regs : 1);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    for (int b = 0; b < n_bins; ++b) {
+        hipError_t e = zero_async(plans[b]->d_active, 2, st);
+        if (e != hipSuccess) return hip_fail(e, "zero queue head");
+    }
+    hipError_t e = cbc::fused_launch(fn, f, waves, st);
+    if (e != hipSuccess) return hip_fail(e, "fused persistent kernel launch");
+    return CBC_OK;
+}
+
 int32_t cbc_selftest_math(int32_t fn, const double* x, const double* y, double* out, int32_t n, void* stream)
 {
     if (fn < CBC_MATH_SINCOS || fn > CBC_MATH_POW1P) return fail(CBC_ERR_INVALID, "unknown function code %d", fn);

@@ -1,0 +1,6 @@
+// Fused persistent kernels over the bins N = 8, 4, 3, 2 with 1 drive harmonic(s), all field types.
+#include "cbc_fused_kernel.h"
+
+namespace cbc {
+const void* fused_pick_8432_k1(int field) { return fused_pick<1, 8, 4, 3, 2>(field); }
+}  // namespace cbc

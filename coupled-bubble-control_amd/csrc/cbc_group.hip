@@ -71,10 +71,48 @@ hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t s
     return hipLaunchKernel(fn, dim3(waves), dim3(64), args, 0, stream);
 }
 
-hipError_t group_attr(int ups, int nk, int field, KernelAttr* out)
+const void* fused_pick_8432_k1(int field);
+const void* fused_pick_8432_k2(int field);
+const void* fused_pick_16_2_k1(int field);
+const void* fused_pick_16_2_k2(int field);
+
+// Bubble-count lists with a fused kernel (largest first, 0 = unused position):
+// the batched uneven ensemble (N = 2, 3, 4, 8) and the large-coupling set mixed
+// with N = 2 systems.
+struct FusedList {
+    int n[kFusedMaxBins];
+    const void* (*pick[2])(int field);  // by harmonic count 1, 2
+};
+static const FusedList kFusedLists[] = {
+    {{8, 4, 3, 2}, {fused_pick_8432_k1, fused_pick_8432_k2}},
+    {{16, 2, 0, 0}, {fused_pick_16_2_k1, fused_pick_16_2_k2}},
+};
+
+const void* fused_pick(const int* ups, int nb, int nk, int field, int* slot)
 {
-    const void* fn = group_pick_any(ups, nk, field);
-    if (!fn) return hipErrorInvalidValue;
+    if (nb < 1 || nb > kFusedMaxBins || nk < 1 || nk > 2) return nullptr;
+    for (const FusedList& l : kFusedLists) {
+        bool covered = true;
+        for (int b = 0; b < nb && covered; ++b) {
+            slot[b] = -1;
+            for (int p = 0; p < kFusedMaxBins; ++p)
+                if (l.n[p] == ups[b]) slot[b] = p;
+            covered = slot[b] >= 0;
+        }
+        if (covered) return l.pick[nk - 1](field);
+    }
+    return nullptr;
+}
+
+hipError_t fused_launch(const void* fn, const FusedArgs& f, int waves, hipStream_t stream)
+{
+    if (!fn || waves <= 0) return hipErrorInvalidValue;
+    void* args[] = {const_cast<FusedArgs*>(&f)};
+    return hipLaunchKernel(fn, dim3(waves), dim3(64), args, 0, stream);
+}
+
+static hipError_t attr_of(const void* fn, KernelAttr* out)
+{
     hipFuncAttributes fa;
     hipError_t e = hipFuncGetAttributes(&fa, fn);
     if (e != hipSuccess) return e;
@@ -83,6 +121,19 @@ hipError_t group_attr(int ups, int nk, int field, KernelAttr* out)
     out->max_threads = fa.maxThreadsPerBlock;
     out->lds_bytes = (int32_t)fa.sharedSizeBytes;
     return hipSuccess;
+}
+
+hipError_t group_attr(int ups, int nk, int field, KernelAttr* out)
+{
+    const void* fn = group_pick_any(ups, nk, field);
+    if (!fn) return hipErrorInvalidValue;
+    return attr_of(fn, out);
+}
+
+hipError_t fused_attr(const void* fn, KernelAttr* out)
+{
+    if (!fn) return hipErrorInvalidValue;
+    return attr_of(fn, out);
 }
 
 }  // namespace cbc

@@ -382,13 +382,15 @@ __device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const G
     }
 }
 
+// The stepper of one wave (one workgroup): plain mode steps the systems of
+// block `block`; persistent mode (queue or ring) takes systems until the queue
+// is drained (or the serving ring idles out).  `lds` holds the wave's S group slots.
 template <int N, int NK, int FIELD>
-__global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
+__device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupLds<N>* lds)
 {
     using Sh = GroupShape<N>;
     constexpr int G = Sh::G;
     constexpr int S = Sh::S;
-    __shared__ GroupLds<N> lds[S];
 
     const int lane = threadIdx.x;
     const int ls = lane / G;
@@ -423,7 +425,7 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
     uint64_t idle_since = 0;
     int idle_rounds = 0;
     if (!persistent) {
-        const int s = blockIdx.x * S + ls;
+        const int s = block * S + ls;
         if (s < a.ns) busy = group_begin<N, NK>(a, s, i, L, R);
         __syncthreads();
         const double g0 = group_event<N>(L, i, busy && R.vu, R.P.rcol, R.y, a.col_factor);
@@ -497,6 +499,13 @@ __global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
             R.vu = false;
         }
     }
+}
+
+template <int N, int NK, int FIELD>
+__global__ void __launch_bounds__(64) group_kernel(const LaunchArgs a)
+{
+    __shared__ GroupLds<N> lds[GroupShape<N>::S];
+    group_run<N, NK, FIELD>(a, blockIdx.x, lds);
 }
 
 // Every build carries the exact event block and tests the mode at run time.

@@ -75,6 +75,14 @@ struct LaunchArgs {
 
 // ServeItem and its option bits: cbc_ring.h
 
+// Fused persistent launch over up to kFusedMaxBins bins (one per bubble count),
+// each a queued launch of its own plan: bin b of the kernel's bubble-count list
+// takes its arguments from bin[b] (queue_len == 0: the bin is absent).
+constexpr int kFusedMaxBins = 4;
+struct FusedArgs {
+    LaunchArgs bin[kFusedMaxBins];
+};
+
 struct KernelAttr {
     int32_t num_regs;
     int32_t local_bytes;
@@ -96,5 +104,12 @@ hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t s
 hipError_t group_attr(int ups, int nk, int field, KernelAttr* out);
 bool group_supported(int ups, int nk, int field);
 int group_systems_per_block(int ups);
+
+// Fused launch: the kernel whose bubble-count list holds every ups[b] (distinct),
+// with slot[b] = the position of ups[b] in that list; null when no list covers
+// the set or no kernel exists for (nk, field).
+const void* fused_pick(const int* ups, int nb, int nk, int field, int* slot);
+hipError_t fused_launch(const void* fn, const FusedArgs& f, int waves, hipStream_t stream);
+hipError_t fused_attr(const void* fn, KernelAttr* out);
 
 }  // namespace cbc

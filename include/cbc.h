@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define CBC_ABI_VERSION 4
+#define CBC_ABI_VERSION 5
 
 /* return codes */
 #define CBC_OK 0
@@ -214,6 +214,28 @@ int32_t cbc_solve_queued(cbc_plan *plan, const cbc_buffers *buf, const int32_t *
  * of systems left ACTIVE is read with cbc_plan_active_count. */
 int32_t cbc_launch_queued(cbc_plan *plan, const cbc_buffers *buf, const int32_t *order, int32_t order_len,
                           int32_t waves, int32_t refill, int32_t kernel_steps, void *stream);
+
+/* The queued launches of n_bins plans with different bubble counts fused into
+ * ONE persistent kernel (no counterpart in the reference, which compiles one
+ * kernel per bubble count, backends/cupy_cuda/kernel_loader.py:49, and so can
+ * integrate only one N per launch).  Bin b is plans[b] over bufs[b], with work
+ * queue orders[b][0..order_lens[b]) and refill threshold refills[b] as in
+ * cbc_launch_queued.  Every wave of the grid drains the bins' queues in turn,
+ * largest bubble count first; each bin's systems step exactly as in its own
+ * queued launch (bit-identical results).  All plans must use the group mapping,
+ * the same harmonic count (1 or 2) and field type, and distinct bubble counts
+ * that one fused kernel covers (cbc_fused_supported: the lists {8, 4, 3, 2} and
+ * {16, 2}); otherwise CBC_ERR_UNSUPPORTED or CBC_ERR_INVALID, with nothing
+ * launched.  `waves` = 0: one wave per SIMD.  Asynchronous and capturable; each
+ * plan's count of systems left ACTIVE is read with cbc_plan_active_count. */
+int32_t cbc_launch_fused(cbc_plan *const *plans, const cbc_buffers *const *bufs, const int32_t *const *orders,
+                         const int32_t *order_lens, const int32_t *refills, int32_t n_bins, int32_t waves,
+                         int32_t kernel_steps, void *stream);
+
+/* 1 if a fused kernel covers the distinct bubble counts ups[0..n_bins) at
+ * harmonic count nk and field type `field` (CBC_FIELD_*), else 0; CBC_ERR_INVALID
+ * for a malformed list. */
+int32_t cbc_fused_supported(const int32_t *ups, int32_t n_bins, int32_t nk, int32_t field);
 
 /* Synchronises `stream` and returns the number of systems the plan's last
  * launch left ACTIVE. */

@@ -74,3 +74,31 @@ def test_plan_create_rejects_bad_event_modes_without_gpu_work():
     assert lib.cbc_plan_create(ctypes.byref(m), ctypes.byref(e), ctypes.byref(s), ctypes.byref(h)) == \
         _abi.CBC_ERR_INVALID
     assert b"event mode" in lib.cbc_last_error()
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="libcbc.so not built")
+def test_fused_kernel_lists_without_gpu_work():
+    """The fused persistent kernels cover the bubble-count lists {8, 4, 3, 2} and {16, 2}
+    (any subset of one list, in any order) at 1 or 2 harmonics and every field type;
+    malformed lists are errors, not answers.  Host-side selection only."""
+    from coupledbubble_control.backends.hip import _abi
+
+    lib = _abi.lib()
+
+    def sup(ups, nk=2, field=_abi.CBC_FIELD_SW_N):
+        arr = (ctypes.c_int32 * max(1, len(ups)))(*ups)
+        return lib.cbc_fused_supported(arr, len(ups), nk, field)
+
+    for ups in ([8, 4, 3, 2], [2, 3, 4, 8], [2, 4], [8], [16, 2], [2, 16], [16], [3, 2]):
+        for nk in (1, 2):
+            for f in (_abi.CBC_FIELD_CONST, _abi.CBC_FIELD_SW_N, _abi.CBC_FIELD_SW_A):
+                assert sup(ups, nk, f) == 1, (ups, nk, f)
+    for ups in ([2, 5], [16, 4], [16, 8, 2], [6]):
+        assert sup(ups) == 0, ups
+    assert sup([2, 4], nk=3) == 0
+    assert sup([2, 4], field=9) == 0
+    assert sup([2, 2]) == _abi.CBC_ERR_INVALID
+    assert b"two bins" in lib.cbc_last_error()
+    assert sup([]) == _abi.CBC_ERR_INVALID
+    assert sup([2, 3, 4, 8, 16]) == _abi.CBC_ERR_INVALID
+    assert lib.cbc_launch_fused(None, None, None, None, None, 1, 0, 1, None) == _abi.CBC_ERR_INVALID

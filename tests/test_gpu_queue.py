@@ -178,3 +178,95 @@ def test_autotune_choices_and_tuned_ensemble(device):
         for k in sa:
             np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
     print(f"  choices { {n: r.choice for n, r in tune.items()} }", flush=True)
+
+
+def _fused_bins(spec, seed):
+    """Workloads of the bins (n, ns): uneven horizons; N = 16 from the large-coupling set."""
+    from coupledbubble_control import workloads
+
+    out = []
+    for n, ns in spec:
+        if n == 16:
+            w = workloads.large_coupling(seed=seed, ns16=ns, ns2=4)[0]
+            w.t_end = np.random.default_rng(seed).uniform(0.05, 0.3, size=ns)
+        else:
+            w = _uneven(n, ns, seed=seed + n, hmax=1.2)
+        out.append(w)
+    return out
+
+
+@pytest.mark.parametrize("spec", [((8, 24), (4, 40), (3, 80), (2, 160)), ((16, 12), (2, 96)), ((2, 100), (4, 40))],
+                         ids=["8-4-3-2", "16-2", "2-4"])
+@pytest.mark.parametrize("budget", [1 << 20, 200], ids=["one-round", "budget-200"])
+def test_fused_equals_per_bin(device, spec, budget):
+    """The bins in one fused persistent kernel give, bit for bit, the per-bin queued
+    launches' results (state, t, h, flags, attempts, event), also when a small attempt
+    budget forces relaunch rounds over the systems still ACTIVE."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.ensemble import BinTuning, MixedEnsemble
+
+    print(f"case: fused vs per-bin {spec} budget={budget}", flush=True)
+    ws = _fused_bins(spec, seed=80)
+    tune = {n: BinTuning("group", refill=1 + (n % 3)) for n, _ in spec}
+    a = [workloads.build_model(w) for w in ws]
+    ea = MixedEnsemble(a, tune, fused="off")
+    ra = ea.solve(kernel_steps=budget)
+    b = [workloads.build_model(w) for w in ws]
+    eb = MixedEnsemble(b, tune, fused="on")
+    rb = eb.solve(kernel_steps=budget)
+    assert eb.fused_active and not ea.fused_active
+    assert ra == rb
+    if budget < 1000:
+        assert ra > 1
+    for ma, mb in zip(a, b):
+        sa, sb = _snapshot(ma), _snapshot(mb)
+        assert (sa["flags"] != 0).all()
+        for k in sa:
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=f"N={ma._execution_spec.ups}: {k}")
+
+
+def test_fused_selection_and_errors(device):
+    """auto fuses only group-mapped bins that a fused kernel covers; "on" maps every bin to
+    groups or refuses the set; the C ABI refuses a lane-mapped plan and leaves it untouched."""
+    import ctypes
+
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+    from coupledbubble_control.backends.hip import _abi
+    from coupledbubble_control.backends.hip import rkck45_coupled_hip_solver as hs
+    from coupledbubble_control.ensemble import BinTuning, MixedEnsemble
+
+    print("case: fused selection", flush=True)
+    w2, w4 = _uneven(2, 64, seed=91, hmax=0.5), _uneven(4, 16, seed=92, hmax=0.5)
+    ens = MixedEnsemble([workloads.build_model(w2), workloads.build_model(w4)],
+                        {2: BinTuning("lane"), 4: BinTuning("group")})
+    ens._prepare()
+    assert not ens.fused_active
+    ens = MixedEnsemble([workloads.build_model(w2), workloads.build_model(w4)],
+                        {2: BinTuning("lane"), 4: BinTuning("group")}, fused="on")
+    ens._prepare()
+    assert ens.fused_active and ens.models[0].runtime.kernel.info().mapping == "group"
+    w5 = _uneven(8, 8, seed=93, hmax=0.5)
+    w5.n = 5
+    w5.R0, w5.X0 = w5.R0[:, :5], w5.X0[:, :5]
+    m5 = workloads.build_model(w5)
+    ens = MixedEnsemble([workloads.build_model(w2), m5])
+    ens._prepare()
+    assert not ens.fused_active
+    with pytest.raises(ValueError, match="no fused kernel"):
+        MixedEnsemble([workloads.build_model(w2), m5], fused="on")._prepare()
+    # C ABI: a lane-mapped plan is refused before anything is launched
+    ml = workloads.build_model(w2, variant=KernelVariant.LANE)
+    ml._ensure_buffers()
+    ml.runtime.solver = ml._make_solver(ml.runtime.backend, ml.runtime.variant)
+    before = _snapshot(ml)
+    order = hs.horizon_order(ml.runtime.device_buffers)
+    with pytest.raises(_abi.CBCError) as ei:
+        hs.launch_fused([ml.runtime.kernel], [ml.runtime.device_buffers], [order], [1])
+    assert ei.value.code == _abi.CBC_ERR_UNSUPPORTED
+    torch.cuda.synchronize()
+    after = _snapshot(ml)
+    for k in before:
+        np.testing.assert_array_equal(before[k], after[k], err_msg=k)
