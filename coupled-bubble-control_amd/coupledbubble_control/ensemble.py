@@ -5,10 +5,18 @@ backends/cupy_cuda/kernel_loader.py:49) and can integrate only one bubble count
 per model.  Here an ensemble holds one CKM1D model per bubble count ("bin", each
 in the reference buffer layout) and integrates all of them together:
 
-* each bin runs one persistent work-queue launch (cbc_launch_queued): lanes or
-  lane groups take that bin's systems longest-horizon-first and refill as
-  systems stop, so uneven horizons and step counts do not idle lanes;
-* the bins' launches are issued on side streams without host
+* each bin's systems go through a persistent work queue: lanes or lane groups
+  take that bin's systems longest-horizon-first and refill as systems stop, so
+  uneven horizons and step counts do not idle lanes;
+* when every bin uses lane groups and a fused kernel covers the bubble counts
+  ({8, 4, 3, 2} or {16, 2}), all bins run in ONE persistent kernel
+  (cbc_launch_fused): each wave drains the bins' queues in turn, largest bubble
+  count first, so the longest attempt chains start first and waves that find a
+  bin drained move on to the next.  32 768-system mixed set: 5.75 s per step
+  against 5.87 s for the per-bin launches below; the bins' results are bit for
+  bit those of their own launches;
+* otherwise (or with fused="off") each bin runs one persistent launch
+  (cbc_launch_queued), issued on side streams without host
   synchronisation between them, so when one bin's queue drains its waves free
   SIMDs for the others (the tails overlap).  A bin's time is bounded below by
   its slowest system's chain of attempts, which grows with N, so the largest
@@ -17,9 +25,9 @@ in the reference buffer layout) and integrates all of them together:
   on one stream run one after the other).  At most three side streams: with
   the current stream they fit the four hardware queues of the device, and a
   launch whose stream shares a hardware queue with another bin's waits for
-  that launch to end.  32 768-system mixed set: 8.0-8.2 s this way; 11.7-12.0 s
-  with one stream per bin and the N = 8 bin on the fourth; the N = 8 bin alone
-  takes 7.8 s;
+  that launch to end.  32 768-system mixed set, in an earlier build with slower
+  N = 8 attempts: 8.0-8.2 s this way, 11.7-12.0 s with one stream per bin and the
+  N = 8 bin on the fourth, the N = 8 bin alone 7.8 s;
 * the launch sequence can be captured once into a HIP graph and replayed;
 * ``autotune`` picks, per bin, the mapping (lane vs group), the refill
   threshold and the lane mapping's workgroup size from short timed trials.
@@ -75,8 +83,8 @@ class MixedEnsemble:
     """One CKM1D model per bubble count, integrated together.  ``fused``: "on" runs
     every bin in one fused persistent kernel (all bins in the group mapping, whatever
     their tuning says; an error if no fused kernel covers the bubble counts), "off"
-    runs one queued launch per bin on side streams, "auto" fuses when a fused kernel
-    covers the bins and every bin's plan uses the group mapping."""
+    runs one queued launch per bin on side streams, "auto" fuses two or more bins when
+    a fused kernel covers them and every bin's plan uses the group mapping."""
 
     def __init__(self, models: Sequence, tuning: Optional[Dict[int, BinTuning]] = None, fused: str = "auto"):
         self.models = list(models)
@@ -100,7 +108,8 @@ class MixedEnsemble:
                 and hs.fused_supported([m._execution_spec.ups for m in self.models], ms.nk, ms.ac))
 
     def _prepare(self):
-        fuse = self.fused != "off" and self._fusable()
+        # one bin alone runs its own kernel (built with that bubble count's flags)
+        fuse = (self.fused == "on" or (self.fused == "auto" and len(self.models) > 1)) and self._fusable()
         if self.fused == "on" and not fuse:
             raise ValueError("no fused kernel covers bubble counts "
                              f"{sorted(m._execution_spec.ups for m in self.models)} at these specs")
