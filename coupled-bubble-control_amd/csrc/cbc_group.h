@@ -1,16 +1,18 @@
 // cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
-// one lane per bubble, 64/TILE systems per wave (one wave per workgroup).
+// one lane per bubble, 64/TILE systems per wave (one wave per workgroup); for
+// N = 9..16 two lanes per bubble (32 per system, 2 systems per wave).
 //
 // This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
 // with a different coupling solve: instead of the block-synchronous Richardson
 // sweep (ckm1d_def.cuh:252-376), lane i owns rows i (radial) and N+i
-// (translational) of the 2N x (2N+1) augmented system in VGPRs and the group
+// (translational) of the 2N x (2N+1) augmented system in VGPRs (for N = 9..16
+// lane i the radial row and lane TILE + i the translational one) and the group
 // runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
-// the largest |a_ik| among rows not yet used (ties: lowest row index), found by a
-// DPP reduction across the group (for N < 16 one max over keys that pack |a|, the
-// row and the sign, where entries closer than 2^-45 relative count as ties); its
-// owner stages it in LDS (double buffered, one wavefront-scope sync per column) and
-// every lane eliminates that column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
+// the largest |a_ik| among rows not yet used (ties: lowest row index), found by one
+// max across the group over keys that pack |a|, the row and the sign (entries
+// closer than 2^-45 relative count as ties); its owner stages it in LDS (double
+// buffered, one wavefront-scope sync per column) and every lane eliminates that
+// column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
 // in LDS once and every lane solves the whole system in registers with the lane
 // mapping's LU (identical results on every lane, one sync instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
@@ -33,15 +35,20 @@ constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2);
 
 template <int N>
 struct GroupShape {
-    static constexpr int G = pow2_ceil(N);   // lanes per system
-    static constexpr int S = 64 / G;         // systems per wave
-    static constexpr int n = 2 * N;          // coupling unknowns
+    static constexpr int T = group_tile(N);   // lanes per bubble set: the buffers' TILE
+    // N = 9..16: two lanes per bubble, lane i (i < T) with its radial coupling row
+    // and lane T + i with its translational row; both hold the bubble's state and
+    // step it identically, and only the first stores it
+    static constexpr bool split = group_lanes(N) != T;
+    static constexpr int G = group_lanes(N);  // lanes per system
+    static constexpr int S = 64 / G;          // systems per wave
+    static constexpr int n = 2 * N;           // coupling unknowns
     static constexpr bool redundant = n <= 8;  // every lane solves the whole system
 };
 
 template <int N>
 struct GroupLds {
-    static constexpr int G = GroupShape<N>::G;
+    static constexpr int G = GroupShape<N>::T;
     static constexpr int n = 2 * N;
     static constexpr bool R = GroupShape<N>::redundant;
     // Most frequently accessed first: ds_read2/ds_write2 address two doubles at
@@ -56,13 +63,14 @@ struct GroupLds {
     double mx[3][N][N];        // mx[c][i][j]: effect of bubble j on bubble i
 };
 
-// Reductions inside a group of G <= 16 lanes (every lane gets the result),
+// Reductions inside a group of G <= 16 lanes (every lane gets the result; the
+// two lanes of a bubble in a 32-lane group hold the same bubble values, so its
+// per-bubble reductions run over each 16-lane half),
 // by DPP row permutations instead of LDS-routed shuffles: partner lane i ^ 1
 // and i ^ 2 (quad_perm), then 7 - i within 8 lanes (row_half_mirror) and
 // 15 - i within 16 (row_mirror).  After the quad steps every lane of a quad
 // holds the quad's result, so each mirror step merges two whole sub-groups.
-// min, OR and the (value, lowest id) maximum do not depend on the order in
-// which elements meet, so the result is a butterfly's (xor partners through
+// min, OR and max do not depend on the order in which elements meet, so the result is a butterfly's (xor partners through
 // LDS-routed shuffles, as these reductions were first written) bit for bit.  The
 // reductions and broadcasts run in wave-uniform code only (all lanes active).
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
@@ -113,39 +121,6 @@ __device__ __forceinline__ bool group_any(bool v)
     return x != 0;
 }
 
-// argmax of (value, id) pairs; ties resolved to the lower id.  The maximum
-// first, then the lowest id among the lanes that hold it.  fmax ignores a NaN
-// operand, so NaN values lose to any number; the maximum is NaN only when every
-// value is NaN, and then every lane is a candidate.  (The Gauss--Jordan column
-// search marks used rows -1: when every unused row is NaN the maximum is -1 and
-// a used row is picked; group_rhs makes that solve's result NaN.)
-template <int G>
-__device__ __forceinline__ void group_argmax(double& v, int& id)
-{
-    double m = v;
-    group_steps<G>([&](auto c) { m = fmax(m, dpp_double<decltype(c)::value>(m)); });
-    int cand = (v == m || m != m) ? id : 0x7fffffff;
-    group_steps<G>([&](auto c) { cand = min(cand, dpp_int<decltype(c)::value>(cand)); });
-    v = m;
-    id = cand;
-}
-
-// group_argmax that also hands every lane the sign bit `neg` of the winning
-// lane's entry: it rides below the row id in the second pass (2 id + neg, so
-// the lowest id still wins), so |a| and the sign give the pivot entry itself
-// without waiting for its row to arrive through LDS.
-template <int G>
-__device__ __forceinline__ void group_argmax_signed(double& v, int& id, bool& neg)
-{
-    double m = v;
-    group_steps<G>([&](auto c) { m = fmax(m, dpp_double<decltype(c)::value>(m)); });
-    int cand = (v == m || m != m) ? (id << 1 | (neg ? 1 : 0)) : 0x7fffffff;
-    group_steps<G>([&](auto c) { cand = min(cand, dpp_int<decltype(c)::value>(cand)); });
-    v = m;
-    id = cand >> 1;
-    neg = (cand & 1) != 0;
-}
-
 // Pivot key of a Gauss--Jordan candidate: |a| with its 7 low bits replaced by
 // (63 - row) << 1 | sign(a).  The largest key of the group is the largest |a|
 // (ties within 2^-45 relative go to the lower row), the lowest row among equal
@@ -169,6 +144,17 @@ __device__ __forceinline__ double group_max(double v)
 {
     group_steps<G>([&](auto c) { v = fmax(v, dpp_double<decltype(c)::value>(v)); });
     return v;
+}
+
+// Maximum over a 32-lane group: the 16-lane maximum of each row, then the two rows
+// of the group exchanged by v_permlane16_swap (with both operands the same value,
+// its two results are the even row's and the odd row's value of the lane's column).
+__device__ __forceinline__ double group_max32(double v)
+{
+    v = group_max<16>(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
 }
 
 // Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
@@ -197,17 +183,11 @@ __device__ __forceinline__ void wave_sync()
 // pairs, these stores made the backend copy both data operands into a fresh
 // register pair first (four extra moves per pair); a volatile LDS store stays a
 // single ds_write_b64 from the row's own registers (N = 8: an attempt 45.0 ->
-// 43.6 us, end states bit for bit the same).  At N = 16 the unmerged stores
-// push the kernel into scratch at the register limit, so it keeps the pairs.
-template <int N>
+// 43.6 us, end states bit for bit the same).
 __device__ __forceinline__ void pivot_store(double* p, double v)
 {
-    if constexpr (N < 16) {
-        typedef __attribute__((address_space(3))) volatile double lds_vdouble;
-        *(lds_vdouble*)p = v;
-    } else {
-        *p = v;
-    }
+    typedef __attribute__((address_space(3))) volatile double lds_vdouble;
+    *(lds_vdouble*)p = v;
 }
 
 template <int... J, typename F>
@@ -262,7 +242,7 @@ __device__ __forceinline__ void group_stage_next(const GroupParams<NK>& P, const
 
 // Coupled RHS for the lane's bubble i.  y, dy = (R, x, U, V) of bubble i.
 template <int N, int NK, int FIELD>
-__device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>& L, int i, bool vu,
+__device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>& L, int i, int half, bool vu,
                                           const GroupStage<NK>& S, const double (&y)[4], double (&dy)[4])
 {
     constexpr int n = 2 * N;
@@ -295,7 +275,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             // no LDS below for pairs (mx is written before the barrier that
             // follows every system load), so no barrier either
         } else {
-            if (vu) {
+            if (vu && half == 0) {
                 L.xh[0][i] = y[1];
                 L.xh[1][i] = o.H0;
                 L.xh[2][i] = o.H1;
@@ -312,7 +292,9 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                 for (int j = 0; j < N; ++j) xh[c][j] = L.xh[c][j];
         }
 
-        // explicit coupling sums and the two matrix rows owned by this lane
+        // explicit coupling sums and the two matrix rows owned by this lane (split
+        // groups: the lane's one row in ar, the radial row for half 0, else the
+        // translational one)
         double ar[n + 1], at[n + 1];
         double rg0 = 0.0, rg1 = 0.0, tng = 0.0, tg2 = 0.0, tg3 = 0.0;
         const int ii = vu ? i : 0;
@@ -336,13 +318,24 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                 tg2 += m1 * (sr2 * H1 + r3 * H3);
                 tg3 += m2 * (sr2 * H1 + r3 * H3);
             }
-            ar[j] = self ? 1.0 : o.G0 * m0 * r * H5;
-            ar[N + j] = self ? 0.0 : 0.5 * o.G0 * m0 * sr2 * H6;
-            at[j] = self ? 0.0 : -m1 * sr2 * H5;
-            at[N + j] = self ? 1.0 : -m1 * r3 * H6;
+            if constexpr (GroupShape<N>::split) {
+                const double cu = half ? -m1 * sr2 : o.G0 * m0 * r;
+                const double cv = half ? -m1 * r3 : 0.5 * o.G0 * m0 * sr2;
+                ar[j] = self ? (half ? 0.0 : 1.0) : cu * H5;
+                ar[N + j] = self ? (half ? 1.0 : 0.0) : cv * H6;
+            } else {
+                ar[j] = self ? 1.0 : o.G0 * m0 * r * H5;
+                ar[N + j] = self ? 0.0 : 0.5 * o.G0 * m0 * sr2 * H6;
+                at[j] = self ? 0.0 : -m1 * sr2 * H5;
+                at[N + j] = self ? 1.0 : -m1 * r3 * H6;
+            }
         }
-        ar[n] = o.dU + (rg0 * o.G0 + rg1 * o.G1);
-        at[n] = o.dV + (tng + tg2 * o.G2 + tg3 * o.G3);
+        if constexpr (GroupShape<N>::split) {
+            ar[n] = half ? o.dV + (tng + tg2 * o.G2 + tg3 * o.G3) : o.dU + (rg0 * o.G0 + rg1 * o.G1);
+        } else {
+            ar[n] = o.dU + (rg0 * o.G0 + rg1 * o.G1);
+            at[n] = o.dV + (tng + tg2 * o.G2 + tg3 * o.G3);
+        }
 
         if constexpr (GroupShape<N>::redundant) {
             // stage the n rows, then every lane solves the whole system in VGPRs
@@ -380,12 +373,48 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             }
             dy[2] = d2;
             dy[3] = d3;
+        } else if constexpr (GroupShape<N>::split) {
+        // Gauss--Jordan with partial pivoting, one row per lane (N = 9..16, 32 lanes):
+        // per column one max over the pivot keys of the group (16-lane DPP steps,
+        // then the group's two rows exchanged), the owner stages its row with its
+        // exact 1 / a_kk in slot k, every lane eliminates the column from its row.
+        // Every solution slot starts as NaN (see below).
+        const int me = half ? N + i : i;
+        if (vu) L.sol[me] = __builtin_nan("");
+        bool used = !vu;
+        int col = 0;
+        double inv_me = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            double* pv = L.piv[k & 1];
+            const double key = used ? -1.0 : pivot_key(ar[k], me);
+            const int row = pivot_key_row(group_max32(key));
+            const bool own = vu && row == me;
+            const double inv_own = rcp_fast(ar[k]);
+            if (own) {
+                pivot_store(pv + k, inv_own);
+#pragma unroll
+                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, ar[j]);
+                used = true;
+                col = k;
+            }
+            wave_sync();
+            const double inv = pv[k];
+            inv_me = own ? inv : inv_me;
+            const double f = own ? 0.0 : ar[k] * inv;
+#pragma unroll
+            for (int j = k + 1; j <= n; ++j) ar[j] -= f * pv[j];
+        }
+        if (vu) L.sol[col] = ar[n] * inv_me;
+        wave_sync();
+        dy[2] = L.sol[ii];
+        dy[3] = L.sol[N + ii];
         } else {
-        // Gauss--Jordan with partial pivoting across the group.  Every solution
-        // slot starts as NaN (the sync above ordered this after the previous
-        // solve's reads): a column that finds no finite unused row (NaN entries)
-        // picks a used row again, so some slot is never written and the
-        // accelerations come out NaN, never stale values of the previous solve.
+        // Gauss--Jordan with partial pivoting across the group, two rows per lane
+        // (N = 5..8).  Every solution slot starts as NaN (the sync above ordered
+        // this after the previous solve's reads): a column that finds no finite
+        // unused row (NaN entries) stages no row, so some slot is never written and
+        // the accelerations come out NaN, never stale values of the previous solve.
         if (vu) {
             L.sol[i] = __builtin_nan("");
             L.sol[N + i] = __builtin_nan("");
@@ -395,56 +424,31 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         double inv_r = 0.0, inv_t = 0.0;
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-            bool own_r, own_t;
             double* pv = L.piv[k & 1];
-            double inv, inv_own = 0.0;
-            if constexpr (N < 16) {
-                // one max over pivot keys; the owner's exact 1 / a_kk travels with
-                // its row in slot k of the buffer (N = 8: an attempt 43.7 -> 41.9 us)
-                const double kr = used_r ? -1.0 : pivot_key(ar[k], i);
-                const double kt = used_t ? -1.0 : pivot_key(at[k], N + i);
-                const int row = pivot_key_row(group_max<GroupShape<N>::G>(fmax(kr, kt)));
-                own_r = vu && row == i;
-                own_t = vu && row == N + i;
-                inv_own = rcp_fast(own_t ? at[k] : ar[k]);
-            } else {
-                // N = 16 sits at the register limit, where the key search's live
-                // ranges spill: the two-pass argmax, whose result gives every lane the
-                // pivot entry, so 1 / a_kk runs alongside the row's LDS trip
-                const double vr = used_r ? -1.0 : fabs(ar[k]);
-                const double vt = used_t ? -1.0 : fabs(at[k]);
-                double v = vr;
-                int id = i;
-                bool neg = __builtin_signbit(ar[k]);
-                if (vt > vr) {
-                    v = vt;
-                    id = N + i;
-                    neg = __builtin_signbit(at[k]);
-                }
-                group_argmax_signed<GroupShape<N>::G>(v, id, neg);
-                // The pivot entry a_kk is the winner's |a| with its sign, bit for bit,
-                // whenever an unused row wins.  (If every unused row is NaN, v is the
-                // -1 mark of a used row and the solve ends NaN whatever inv is.)
-                inv = rcp_fast(neg ? -v : v);
-                own_r = vu && id == i;
-                own_t = vu && id == N + i;
-            }
+            // one max over pivot keys; the owner's exact 1 / a_kk travels with
+            // its row in slot k of the buffer (N = 8: an attempt 43.7 -> 41.9 us)
+            const double kr = used_r ? -1.0 : pivot_key(ar[k], i);
+            const double kt = used_t ? -1.0 : pivot_key(at[k], N + i);
+            const int row = pivot_key_row(group_max<GroupShape<N>::G>(fmax(kr, kt)));
+            const bool own_r = vu && row == i;
+            const bool own_t = vu && row == N + i;
+            const double inv_own = rcp_fast(own_t ? at[k] : ar[k]);
             if (own_r) {
-                if constexpr (N < 16) pivot_store<N>(pv + k, inv_own);
+                pivot_store(pv + k, inv_own);
 #pragma unroll
-                for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, ar[j]);
+                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, ar[j]);
                 used_r = true;
                 col_r = k;
             }
             if (own_t) {
-                if constexpr (N < 16) pivot_store<N>(pv + k, inv_own);
+                pivot_store(pv + k, inv_own);
 #pragma unroll
-                for (int j = k + 1; j <= n; ++j) pivot_store<N>(pv + j, at[j]);
+                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, at[j]);
                 used_t = true;
                 col_t = k;
             }
             wave_sync();
-            if constexpr (N < 16) inv = pv[k];
+            const double inv = pv[k];
             inv_r = own_r ? inv : inv_r;
             inv_t = own_t ? inv : inv_t;
             const double fr = own_r ? 0.0 : ar[k] * inv;
@@ -469,13 +473,13 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
 
 // Collision gap of the system, identical on every lane of the group.
 template <int N>
-__device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, double rcol, const double (&y)[4],
-                                              double colf)
+__device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool wr, bool vu, double rcol,
+                                              const double (&y)[4], double colf)
 {
     if constexpr (N == 1) {
         return 1e30;
     } else {
-        if (vu) {
+        if (vu && wr) {
             L.ev[0][i] = y[0] * rcol;
             L.ev[1][i] = y[1];
         }
@@ -486,7 +490,7 @@ __device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, do
             gap = fabs(L.ev[1][i] - L.ev[1][i + 1]) - (ri + rj) * colf;
         }
         wave_sync();  // ev is rewritten by the next call
-        return group_min<GroupShape<N>::G>(gap);
+        return group_min<GroupShape<N>::T>(gap);
     }
 }
 
@@ -495,7 +499,7 @@ __device__ __forceinline__ double group_event(GroupLds<N>& L, int i, bool vu, do
 // iterations (the gap evaluation syncs and reduces across the wave); a group that converged keeps
 // its iterate, so the result is the early-exit one of the lane mapping.
 template <int N>
-__device__ double group_event_root(GroupLds<N>& L, int i, bool vu, double rcol, const double (&y0)[4],
+__device__ double group_event_root(GroupLds<N>& L, int i, bool wr, bool vu, double rcol, const double (&y0)[4],
                                    const double (&y1)[4], double h, double colf, double g0, double g1)
 {
     Illinois st{0.0, g0, 1.0, g1, false};
@@ -506,7 +510,7 @@ __device__ double group_event_root(GroupLds<N>& L, int i, bool vu, double rcol, 
         yc[1] = hermite3(y0[1], y1[1], h * y0[3], h * y1[3], c);
         yc[2] = 0.0;
         yc[3] = 0.0;
-        const double g = group_event<N>(L, i, vu, rcol, yc, colf);
+        const double g = group_event<N>(L, i, wr, vu, rcol, yc, colf);
         illinois_update(st, c, g);
     }
     return st.b;

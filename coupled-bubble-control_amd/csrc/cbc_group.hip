@@ -20,13 +20,6 @@ const void* group_pick_n14(int nk, int field);
 const void* group_pick_n15(int nk, int field);
 const void* group_pick_n16(int nk, int field);
 
-static int pow2_up(int v)
-{
-    int t = 1;
-    while (t < v) t <<= 1;
-    return t;
-}
-
 static const void* group_pick_any(int ups, int nk, int field)
 {
     switch (ups) {
@@ -51,7 +44,7 @@ static const void* group_pick_any(int ups, int nk, int field)
 
 bool group_supported(int ups, int nk, int field) { return group_pick_any(ups, nk, field) != nullptr; }
 
-int group_systems_per_block(int ups) { return 64 / pow2_up(ups); }
+int group_systems_per_block(int ups) { return 64 / group_lanes(ups); }
 
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream)
 {

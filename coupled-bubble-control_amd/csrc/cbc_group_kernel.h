@@ -8,7 +8,10 @@
 // every attempt together (the RHS syncs and reduces across the wave), so lanes of idle groups
 // compute but never store.
 //
-// Plain mode: group = system blockIdx.x * S + group index.
+// Plain mode: group = system blockIdx.x * S + group index.  Lane i of a group
+// carries bubble i; in the 32-lane groups of N = 9..16, lanes i and 16 + i both
+// carry bubble i (its radial and its translational coupling row) and step it
+// identically, and lane i alone stores its results.
 // Persistent mode: free groups take systems from the work queue (one atomic per
 // refill by the wave, consecutive slots to free groups in lane order) once at
 // least `refill` groups are free, or all are.
@@ -37,7 +40,7 @@ struct GroupRun {
 // system is active; a system finished at load gets its flag written.  A serving
 // item (`it` != null) starts at tau = 0 as in lane_begin (cbc_lane.hip).
 template <int N, int NK>
-__device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, GroupLds<N>& L,
+__device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, bool wr, GroupLds<N>& L,
                                             GroupRun<N, NK>& R, const ServeItem* it = nullptr)
 {
     const size_t NS = (size_t)a.ns;
@@ -73,7 +76,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         active = false;
     }
     if (!active) {
-        if (i == 0) a.flags[s] = R.flag;
+        if (i == 0 && wr) a.flags[s] = R.flag;
         R.vu = false;
         return false;
     }
@@ -110,7 +113,7 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         }
     }
     // coupling constants: coupling[c][j][s][i] is the effect of unit j on unit i
-    if (R.vu) {
+    if (R.vu && wr) {
         const size_t cstride = (size_t)a.ups * NST;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -147,8 +150,8 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
         R.didx = a.dense_index[s];
         R.ddt = (a.ndo == 1) ? (R.t_end - tb) : (R.t_end - tb) / (a.ndo - 1);
         if (R.didx < a.ndo && R.dnext <= R.t) {
-            if (i == 0) a.dense_time[(size_t)R.didx * NS + s] = R.t;
-            if (R.vu)
+            if (i == 0 && wr) a.dense_time[(size_t)R.didx * NS + s] = R.t;
+            if (R.vu && wr)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     a.dense_state[(((size_t)R.didx * 4 + c) * NS + s) * TILE + i] = R.y[c];
@@ -177,10 +180,11 @@ __device__ __forceinline__ bool group_begin(const LaunchArgs& a, int s, int i, G
 // marks groups with a live system; returns true for a busy group whose system
 // stopped this attempt.
 template <int N, int NK, int FIELD>
-__device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupLds<N>& L, GroupRun<N, NK>& R,
-                                              bool busy)
+__device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, int half, GroupLds<N>& L,
+                                              GroupRun<N, NK>& R, bool busy)
 {
-    constexpr int G = GroupShape<N>::G;
+    constexpr int T = GroupShape<N>::T;  // per-bubble reductions: over one half of a split group
+    const bool wr = half == 0;
     const size_t NS = (size_t)a.ns;
     const size_t TILE = (size_t)a.tile;
     double& t = R.t;
@@ -223,7 +227,7 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
             }
             group_stage_next<NK, FIELD>(R.P, B, rxb, t, y, t + kCkA[q] * h, x, vu, S);
         }
-        group_rhs<N, NK, FIELD>(R.P, L, i, vu, S, x, k);
+        group_rhs<N, NK, FIELD>(R.P, L, i, half, vu, S, x, k);
 #pragma unroll
         for (int r = 1; r < 6; ++r) {
             if (r > q) {
@@ -243,7 +247,7 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     double ynew[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) ynew[c] = y[c] + h * ysum[c];
-    const double g_new = group_event<N>(L, i, vu, R.P.rcol, ynew, a.col_factor);
+    const double g_new = group_event<N>(L, i, wr, vu, R.P.rcol, ynew, a.col_factor);
 
     // error ratio of the lane's bubble, reduced over the group
     double ratio_u = 1.0e30;
@@ -258,14 +262,14 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
             if (r < ratio_u) ratio_u = r;
         }
     }
-    const double ratio = group_min<G>(ratio_u);
-    const bool bad = group_any<G>(bad_u);
+    const double ratio = group_min<T>(ratio_u);
+    const bool bad = group_any<T>(bad_u);
     // exact event mode: the gap's root on the step, for groups that cross
     double theta = 1.0;
     if (a.event_exact) {
         const bool need = busy && R.g_old * g_new < 0.0 && !(fabs(g_new) < a.etol);
         if (__any(need))
-            theta = group_event_root<N>(L, i, vu, R.P.rcol, y, ynew, h, a.col_factor, R.g_old, g_new);
+            theta = group_event_root<N>(L, i, wr, vu, R.P.rcol, y, ynew, h, a.col_factor, R.g_old, g_new);
     }
     if (!busy) return false;
 
@@ -309,8 +313,8 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
         h = h * fac;
     }
     if (a.ndo > 0 && R.didx < a.ndo && R.dnext <= t) {
-        if (i == 0) a.dense_time[(size_t)R.didx * NS + s] = t;
-        if (R.vu)
+        if (i == 0 && wr) a.dense_time[(size_t)R.didx * NS + s] = t;
+        if (R.vu && wr)
 #pragma unroll
             for (int c = 0; c < 4; ++c) a.dense_state[(((size_t)R.didx * 4 + c) * NS + s) * TILE + i] = y[c];
         ++R.didx;
@@ -318,7 +322,7 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
     }
     if (a.period > 0.0 && t >= R.wend) {
         const int widx = R.wcur - a.first_window;
-        if (R.vu && widx >= 0 && widx < a.n_windows) {
+        if (R.vu && wr && widx >= 0 && widx < a.n_windows) {
             a.max_r[((size_t)widx * NS + s) * TILE + i] = R.wmax;
             a.mean_x[((size_t)widx * NS + s) * TILE + i] = R.wsum / a.period;
         }
@@ -339,8 +343,9 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, GroupL
 }
 
 template <int N, int NK>
-__device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, const GroupRun<N, NK>& R)
+__device__ __forceinline__ void group_finish(const LaunchArgs& a, int i, bool wr, const GroupRun<N, NK>& R)
 {
+    if (!wr) return;  // the second lane of a bubble (split groups) stores nothing
     const size_t NS = (size_t)a.ns;
     const size_t TILE = (size_t)a.tile;
     const size_t NST = NS * TILE;
@@ -394,7 +399,10 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
 
     const int lane = threadIdx.x;
     const int ls = lane / G;
-    const int i = lane % G;
+    const int gi = lane % G;      // lane within the group
+    const int i = gi % Sh::T;     // its bubble
+    const int half = gi / Sh::T;  // split groups: 0 radial-row lane, 1 translational-row lane
+    const bool wr = half == 0;    // the lane that stores the bubble's results
     GroupLds<N>& L = lds[ls];
     GroupRun<N, NK> R;
     R.vu = false;
@@ -426,14 +434,14 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
     int idle_rounds = 0;
     if (!persistent) {
         const int s = block * S + ls;
-        if (s < a.ns) busy = group_begin<N, NK>(a, s, i, L, R);
+        if (s < a.ns) busy = group_begin<N, NK>(a, s, i, wr, L, R);
         __syncthreads();
-        const double g0 = group_event<N>(L, i, busy && R.vu, R.P.rcol, R.y, a.col_factor);
+        const double g0 = group_event<N>(L, i, wr, busy && R.vu, R.P.rcol, R.y, a.col_factor);
         if (busy) R.g_old = g0;
     }
     while (true) {
         if (!drained) {
-            const bool free_leader = !busy && i == 0;
+            const bool free_leader = !busy && gi == 0;
             const uint64_t freemask = __ballot(free_leader);
             const int nfree = __popcll(freemask);
             const bool any_busy = __any(busy);
@@ -455,10 +463,10 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
                     if (!busy && slot >= 0) {
                         const ServeItem it = serve_item(a, base + (uint32_t)slot);
                         if (it.sys >= 0 && it.sys < a.ns) {  // the host validates; never trust the ring
-                            fresh = group_begin<N, NK>(a, it.sys, i, L, R, &it);
+                            fresh = group_begin<N, NK>(a, it.sys, i, wr, L, R, &it);
                             R.seq = it.seq;
                             used = 0;
-                            if (!fresh && i == 0) serve_publish_flag(a, it.sys, it.seq, R.flag);
+                            if (!fresh && gi == 0) serve_publish_flag(a, it.sys, it.seq, R.flag);
                         }
                     }
                 } else {
@@ -470,12 +478,12 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
                     idx = __shfl(idx, ls * G);  // group leader's slot to the whole group
                     const int q = (!busy && idx < (uint32_t)a.queue_len) ? a.queue[idx] : -1;
                     if (q >= 0 && q < a.ns) {  // malformed entries are skipped
-                        fresh = group_begin<N, NK>(a, q, i, L, R);
+                        fresh = group_begin<N, NK>(a, q, i, wr, L, R);
                         used = 0;
                     }
                 }
                 __syncthreads();
-                const double g0 = group_event<N>(L, i, fresh && R.vu, R.P.rcol, R.y, a.col_factor);
+                const double g0 = group_event<N>(L, i, wr, fresh && R.vu, R.P.rcol, R.y, a.col_factor);
                 if (fresh) {
                     R.g_old = g0;
                     busy = true;
@@ -492,9 +500,9 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
         }
         idle_since = 0;
         ++since_poll;
-        const bool stop = group_attempt<N, NK, FIELD>(a, i, L, R, busy);
+        const bool stop = group_attempt<N, NK, FIELD>(a, i, half, L, R, busy);
         if (busy && (stop || ++used >= a.kernel_steps)) {
-            group_finish<N, NK>(a, i, R);
+            group_finish<N, NK>(a, i, wr, R);
             busy = false;
             R.vu = false;
         }

@@ -49,8 +49,8 @@ def test_group_mapping_parity(device, n, ns, periods):
 @pytest.mark.parametrize("n,ns,periods", [(5, 48, 1.0), (6, 40, 1.0), (7, 32, 1.0), (9, 24, 0.5), (10, 16, 0.5),
                                           (11, 16, 0.5), (12, 16, 0.5), (13, 12, 0.5), (14, 12, 0.5), (15, 12, 0.5)])
 def test_group_mapping_other_bubble_counts(device, n, ns, periods):
-    """Bubble counts that are not powers of two run in groups of next_pow2(N) lanes with
-    padding lanes in the Gauss--Jordan solve (N = 5..7 in 8 lanes, 9..15 in 16)."""
+    """Bubble counts that are not powers of two run in groups with padding lanes in the
+    Gauss--Jordan solve (N = 5..7 in 8 lanes; 9..15 in 32, two lanes per bubble)."""
     from coupledbubble_control.backends import KernelVariant
 
     print(f"case: group mapping N={n} {ns} systems {periods} periods", flush=True)
@@ -123,6 +123,40 @@ def test_group_dense_and_windows(device):
     w3 = _mixed(3, 32, 1.0)
     m3, ref3, _ = _run_pair(w3, ndo=8, variant=KernelVariant.GROUP)
     np.testing.assert_array_equal(m3.runtime.host_buffers.dense.dense_index, ref3["dense_index"])
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_split_groups_dense_and_windows(device, n):
+    """Two lanes per bubble (N = 9..16): only the first lane of a bubble stores, so dense
+    rows, window statistics and end states match the oracle as for the other mappings."""
+    import torch
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+    from oracle import ckm1d_oracle as O
+
+    print(f"case: split group N={n} windows, 12 systems, 0.6 periods", flush=True)
+    w = _any_n(n, 12, 0.6)
+    m = workloads.build_model(w, variant=KernelVariant.GROUP)
+    ref = O.host_buffers_dict(m)
+    out = O.solve_periodic(O.spec_from_model(m), ref, 0.2, 1, 2)
+    max_r, mean_x = m.solve_periodic(period=0.2, first_window=1, n_windows=2)
+    torch.cuda.synchronize()
+    flags = m.runtime.device_buffers.status.status_flags.cpu().numpy()
+    np.testing.assert_array_equal(flags, ref["status_flags"])
+    ok = flags == 1
+    mr = max_r.cpu().numpy()[:, ok, :n]
+    rel = np.abs(mr - out["max_r"][:, ok, :n]) / out["max_r"][:, ok, :n]
+    assert np.mean(rel <= 1e-6) >= 0.95 and rel.max() <= 2e-4
+    np.testing.assert_allclose(mean_x.cpu().numpy()[:, ok, :n], out["mean_x"][:, ok, :n], rtol=1e-6, atol=1e-12)
+    assert np.all(max_r.cpu().numpy()[:, :, n:] == 0.0), "padding columns must stay untouched"
+
+    print(f"case: split group N={n} dense output ndo=5", flush=True)
+    md, refd, cnt = _run_pair(_any_n(n, 12, 0.4), ndo=5, variant=KernelVariant.GROUP)
+    np.testing.assert_array_equal(md.runtime.host_buffers.dense.dense_index, refd["dense_index"])
+    np.testing.assert_array_equal(md.runtime.host_buffers.status.status_flags, refd["status_flags"])
+    _assert_states(md, refd, n)
+    _assert_counts(md, cnt)
 
 
 def test_auto_mapping_for_two_bubbles(device):

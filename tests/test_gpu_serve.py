@@ -68,10 +68,11 @@ def _rows(m, ids):
 
 @pytest.mark.parametrize("n,ns,variant,exact", [(2, 256, None, False), (4, 128, None, False),
                                                (2, 128, "group", False), (4, 128, None, True),
-                                               (8, 64, None, True)])
+                                               (8, 64, None, True), (12, 24, None, False)])
 def test_serve_equals_plain_bitwise(n, ns, variant, exact):
     """A served step equals a plain launch of the same step bit for bit; also in the exact
-    event mode (N = 8: the Gauss--Jordan kernel's exact build on the serving path)."""
+    event mode (N = 8: the Gauss--Jordan kernel's exact build on the serving path) and
+    with two lanes per bubble (N = 12)."""
     import dataclasses
 
     from coupledbubble_control.backends.hip.serve import ServeOptions, ServeQueue
