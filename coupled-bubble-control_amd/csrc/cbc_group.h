@@ -1,20 +1,21 @@
 // cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
-// one lane per bubble, 64/TILE systems per wave (one wave per workgroup); for
-// N = 9..16 two lanes per bubble (32 per system, 2 systems per wave).
+// one lane per bubble, 64/TILE systems per wave (one wave per workgroup), for
+// N <= 4; for N = 5..16 two lanes per bubble (2 TILE per system).
 //
 // This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
 // with a different coupling solve: instead of the block-synchronous Richardson
-// sweep (ckm1d_def.cuh:252-376), lane i owns rows i (radial) and N+i
-// (translational) of the 2N x (2N+1) augmented system in VGPRs (for N = 9..16
-// lane i the radial row and lane TILE + i the translational one) and the group
-// runs Gauss--Jordan with partial pivoting: per column the pivot is the row with
+// sweep (ckm1d_def.cuh:252-376), for N = 5..16 lane i of a group owns row i
+// (radial) and lane TILE + i row N + i (translational) of the 2N x (2N+1)
+// augmented system in VGPRs, both carrying bubble i, and the group runs
+// Gauss--Jordan with partial pivoting: per column the pivot is the row with
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by one
 // max across the group over keys that pack |a|, the row and the sign (entries
 // closer than 2^-45 relative count as ties); its owner stages it in LDS (double
 // buffered, one wavefront-scope sync per column) and every lane eliminates that
-// column from its rows.  For N <= 4 (n <= 8) the group instead stages its n rows
-// in LDS once and every lane solves the whole system in registers with the lane
-// mapping's LU (identical results on every lane, one sync instead of n).
+// column from its row.  For N <= 4 (n <= 8) lane i builds both rows of bubble i,
+// the group stages its n rows in LDS once and every lane solves the whole system
+// in registers with the lane mapping's LU (identical results on every lane, one
+// sync instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
 // coupling constants mx are staged in LDS per system; pairs (N = 2) pass the
 // bubble data and the rows by DPP broadcasts instead.  All lanes of a group
@@ -36,7 +37,7 @@ constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2);
 template <int N>
 struct GroupShape {
     static constexpr int T = group_tile(N);   // lanes per bubble set: the buffers' TILE
-    // N = 9..16: two lanes per bubble, lane i (i < T) with its radial coupling row
+    // N = 5..16: two lanes per bubble, lane i (i < T) with its radial coupling row
     // and lane T + i with its translational row; both hold the bubble's state and
     // step it identically, and only the first stores it
     static constexpr bool split = group_lanes(N) != T;
@@ -146,15 +147,21 @@ __device__ __forceinline__ double group_max(double v)
     return v;
 }
 
-// Maximum over a 32-lane group: the 16-lane maximum of each row, then the two rows
-// of the group exchanged by v_permlane16_swap (with both operands the same value,
-// its two results are the even row's and the odd row's value of the lane's column).
-__device__ __forceinline__ double group_max32(double v)
+// Maximum over a group of G lanes; a 32-lane group takes the 16-lane maximum of
+// each row, then exchanges its two rows by v_permlane16_swap (with both operands
+// the same value, its two results are the even row's and the odd row's value of
+// the lane's column).
+template <int G>
+__device__ __forceinline__ double group_max_wide(double v)
 {
-    v = group_max<16>(v);
-    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
-    return fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
+    if constexpr (G == 32) {
+        v = group_max<16>(v);
+        const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+        return fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
+    } else {
+        return group_max<G>(v);
+    }
 }
 
 // Value of lane j of the lane's group, for groups of 2 or 4 lanes (inside one
@@ -373,12 +380,16 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             }
             dy[2] = d2;
             dy[3] = d3;
-        } else if constexpr (GroupShape<N>::split) {
-        // Gauss--Jordan with partial pivoting, one row per lane (N = 9..16, 32 lanes):
-        // per column one max over the pivot keys of the group (16-lane DPP steps,
-        // then the group's two rows exchanged), the owner stages its row with its
-        // exact 1 / a_kk in slot k, every lane eliminates the column from its row.
-        // Every solution slot starts as NaN (see below).
+        } else {
+        // Gauss--Jordan with partial pivoting, one row per lane (N = 5..16, 16 or 32
+        // lanes): per column one max over the pivot keys of the group (DPP steps
+        // within the 16-lane row, then for 32 lanes the group's two rows exchanged),
+        // the owner stages its row with its exact 1 / a_kk in slot k, every lane
+        // eliminates the column from its row.  Every solution slot starts as NaN
+        // (the sync before this solve ordered it after the previous solve's
+        // reads): a column that finds no finite unused row (NaN entries) stages no
+        // row, so some slot is never written and the accelerations come out NaN,
+        // never stale values of the previous solve.
         const int me = half ? N + i : i;
         if (vu) L.sol[me] = __builtin_nan("");
         bool used = !vu;
@@ -388,7 +399,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         for (int k = 0; k < n; ++k) {
             double* pv = L.piv[k & 1];
             const double key = used ? -1.0 : pivot_key(ar[k], me);
-            const int row = pivot_key_row(group_max32(key));
+            const int row = pivot_key_row(group_max_wide<GroupShape<N>::G>(key));
             const bool own = vu && row == me;
             const double inv_own = rcp_fast(ar[k]);
             if (own) {
@@ -406,64 +417,6 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             for (int j = k + 1; j <= n; ++j) ar[j] -= f * pv[j];
         }
         if (vu) L.sol[col] = ar[n] * inv_me;
-        wave_sync();
-        dy[2] = L.sol[ii];
-        dy[3] = L.sol[N + ii];
-        } else {
-        // Gauss--Jordan with partial pivoting across the group, two rows per lane
-        // (N = 5..8).  Every solution slot starts as NaN (the sync above ordered
-        // this after the previous solve's reads): a column that finds no finite
-        // unused row (NaN entries) stages no row, so some slot is never written and
-        // the accelerations come out NaN, never stale values of the previous solve.
-        if (vu) {
-            L.sol[i] = __builtin_nan("");
-            L.sol[N + i] = __builtin_nan("");
-        }
-        bool used_r = !vu, used_t = !vu;
-        int col_r = 0, col_t = 0;
-        double inv_r = 0.0, inv_t = 0.0;
-#pragma unroll
-        for (int k = 0; k < n; ++k) {
-            double* pv = L.piv[k & 1];
-            // one max over pivot keys; the owner's exact 1 / a_kk travels with
-            // its row in slot k of the buffer (N = 8: an attempt 43.7 -> 41.9 us)
-            const double kr = used_r ? -1.0 : pivot_key(ar[k], i);
-            const double kt = used_t ? -1.0 : pivot_key(at[k], N + i);
-            const int row = pivot_key_row(group_max<GroupShape<N>::G>(fmax(kr, kt)));
-            const bool own_r = vu && row == i;
-            const bool own_t = vu && row == N + i;
-            const double inv_own = rcp_fast(own_t ? at[k] : ar[k]);
-            if (own_r) {
-                pivot_store(pv + k, inv_own);
-#pragma unroll
-                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, ar[j]);
-                used_r = true;
-                col_r = k;
-            }
-            if (own_t) {
-                pivot_store(pv + k, inv_own);
-#pragma unroll
-                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, at[j]);
-                used_t = true;
-                col_t = k;
-            }
-            wave_sync();
-            const double inv = pv[k];
-            inv_r = own_r ? inv : inv_r;
-            inv_t = own_t ? inv : inv_t;
-            const double fr = own_r ? 0.0 : ar[k] * inv;
-            const double ft = own_t ? 0.0 : at[k] * inv;
-#pragma unroll
-            for (int j = k + 1; j <= n; ++j) {
-                const double p = pv[j];
-                ar[j] -= fr * p;
-                at[j] -= ft * p;
-            }
-        }
-        if (vu) {
-            L.sol[col_r] = ar[n] * inv_r;
-            L.sol[col_t] = at[n] * inv_t;
-        }
         wave_sync();
         dy[2] = L.sol[ii];
         dy[3] = L.sol[N + ii];
