@@ -39,8 +39,11 @@ struct GroupShape {
     static constexpr int T = group_tile(N);   // lanes per bubble set: the buffers' TILE
     // N = 5..16: two lanes per bubble, lane i (i < T) with its radial coupling row
     // and lane T + i with its translational row; both hold the bubble's state and
-    // step it identically, and only the first stores it
+    // step it identically, and only the first stores it.  N = 9..16 (`wave`): one
+    // system per wave, lanes 32 + i and 48 + i carry bubble i once more (same
+    // state, no part in the solve), so the pivot's owner is one lane of the wave
     static constexpr bool split = group_lanes(N) != T;
+    static constexpr bool wave = group_lanes(N) == 64;
     static constexpr int G = group_lanes(N);  // lanes per system
     static constexpr int S = 64 / G;          // systems per wave
     static constexpr int n = 2 * N;           // coupling unknowns
@@ -197,6 +200,14 @@ __device__ __forceinline__ void pivot_store(double* p, double v)
     *(lds_vdouble*)p = v;
 }
 
+// Value of v in lane `lane` (wave-uniform) for every lane.
+__device__ __forceinline__ double readlane_double(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 template <int... J, typename F>
 __device__ __forceinline__ void for_lanes(std::integer_sequence<int, J...>, F f)
 {
@@ -326,10 +337,11 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
                 tg3 += m2 * (sr2 * H1 + r3 * H3);
             }
             if constexpr (GroupShape<N>::split) {
-                const double cu = half ? -m1 * sr2 : o.G0 * m0 * r;
-                const double cv = half ? -m1 * r3 : 0.5 * o.G0 * m0 * sr2;
-                ar[j] = self ? (half ? 0.0 : 1.0) : cu * H5;
-                ar[N + j] = self ? (half ? 1.0 : 0.0) : cv * H6;
+                const bool tr = (half & 1) != 0;   // translational-row lane
+                const double cu = tr ? -m1 * sr2 : o.G0 * m0 * r;
+                const double cv = tr ? -m1 * r3 : 0.5 * o.G0 * m0 * sr2;
+                ar[j] = self ? (tr ? 0.0 : 1.0) : cu * H5;
+                ar[N + j] = self ? (tr ? 1.0 : 0.0) : cv * H6;
             } else {
                 ar[j] = self ? 1.0 : o.G0 * m0 * r * H5;
                 ar[N + j] = self ? 0.0 : 0.5 * o.G0 * m0 * sr2 * H6;
@@ -338,7 +350,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             }
         }
         if constexpr (GroupShape<N>::split) {
-            ar[n] = half ? o.dV + (tng + tg2 * o.G2 + tg3 * o.G3) : o.dU + (rg0 * o.G0 + rg1 * o.G1);
+            ar[n] = (half & 1) ? o.dV + (tng + tg2 * o.G2 + tg3 * o.G3) : o.dU + (rg0 * o.G0 + rg1 * o.G1);
         } else {
             ar[n] = o.dU + (rg0 * o.G0 + rg1 * o.G1);
             at[n] = o.dV + (tng + tg2 * o.G2 + tg3 * o.G3);
@@ -380,8 +392,41 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             }
             dy[2] = d2;
             dy[3] = d3;
+        } else if constexpr (GroupShape<N>::wave) {
+        // Gauss--Jordan with partial pivoting, one row per lane, one system per wave
+        // (N = 9..16): as below, but the pivot's owner is the same lane for the whole
+        // wave, so its 1 / a_kk and row reach every lane by v_readlane (through
+        // SGPRs) instead of an LDS round trip and a sync per column; the arithmetic
+        // is the same.  Lanes 32..63 are no candidates (key -1).
+        const bool cand = vu && half < 2;
+        const int me = (half & 1) ? N + i : i;
+        if (cand) L.sol[me] = __builtin_nan("");
+        bool used = !cand;
+        int col = 0;
+        double inv_me = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+            const double key = used ? -1.0 : pivot_key(ar[k], me);
+            // every candidate lane holds the same maximum; lane 0 is one of them
+            const int row = __builtin_amdgcn_readfirstlane(pivot_key_row(group_max_wide<32>(key)));
+            const int src = row < N ? row : (row < 2 * N ? 16 + row - N : 0);  // owner lane (63: none)
+            const bool own = cand && row == me;
+            const double inv = readlane_double(rcp_fast(ar[k]), src);
+            inv_me = own ? inv : inv_me;
+            if (own) {
+                used = true;
+                col = k;
+            }
+            const double f = own ? 0.0 : ar[k] * inv;
+#pragma unroll
+            for (int j = k + 1; j <= n; ++j) ar[j] -= f * readlane_double(ar[j], src);
+        }
+        if (cand) L.sol[col] = ar[n] * inv_me;
+        wave_sync();
+        dy[2] = L.sol[ii];
+        dy[3] = L.sol[N + ii];
         } else {
-        // Gauss--Jordan with partial pivoting, one row per lane (N = 5..16, 16 or 32
+        // Gauss--Jordan with partial pivoting, one row per lane (N = 5..8, 16
         // lanes): per column one max over the pivot keys of the group (DPP steps
         // within the 16-lane row, then for 32 lanes the group's two rows exchanged),
         // the owner stages its row with its exact 1 / a_kk in slot k, every lane

@@ -100,10 +100,15 @@ bool lane_supported(int ups, int nk, int field);
 
 // Group mapping, one wave per workgroup: the buffers' TILE = next_pow2(ups) lanes
 // per system, one per bubble (ups <= 4, whose lanes each solve the whole coupling
-// system); for ups = 5..16 two lanes per bubble (16 or 32 per system), one per
-// coupling row.
+// system); for ups = 5..8 two lanes per bubble (16 per system), one per coupling
+// row; for ups = 9..16 one system per wave (64 lanes: the two row lanes of each
+// bubble, and a copy of them that carries the bubble without taking part in the
+// solve).
 constexpr int group_tile(int ups) { return ups <= 1 ? 1 : 2 * group_tile((ups + 1) / 2); }
-constexpr int group_lanes(int ups) { return group_tile(ups) >= 8 ? 2 * group_tile(ups) : group_tile(ups); }
+constexpr int group_lanes(int ups)
+{
+    return group_tile(ups) == 16 ? 64 : group_tile(ups) == 8 ? 16 : group_tile(ups);
+}
 
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
 hipError_t group_launch_persistent(const LaunchArgs& a, int waves, hipStream_t stream);
