@@ -1,6 +1,7 @@
 // cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
 // one lane per bubble, 64/TILE systems per wave (one wave per workgroup), for
-// N <= 4; for N = 5..16 two lanes per bubble (2 TILE per system).
+// N <= 4; for N = 5..16 two lanes per bubble (16 lanes per system for N = 5..8,
+// one system per wave for N = 9..16).
 //
 // This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
 // with a different coupling solve: instead of the block-synchronous Richardson
@@ -12,7 +13,8 @@
 // max across the group over keys that pack |a|, the row and the sign (entries
 // closer than 2^-45 relative count as ties); its owner stages it in LDS (double
 // buffered, one wavefront-scope sync per column) and every lane eliminates that
-// column from its row.  For N <= 4 (n <= 8) lane i builds both rows of bubble i,
+// column from its row (for N = 9..16, one system per wave, the owner's row
+// reaches the lanes by v_readlane instead).  For N <= 4 (n <= 8) lane i builds both rows of bubble i,
 // the group stages its n rows in LDS once and every lane solves the whole system
 // in registers with the lane mapping's LU (identical results on every lane, one
 // sync instead of n).

@@ -58,7 +58,7 @@ extern "C" {
 /* system-to-hardware mapping */
 #define CBC_MAP_AUTO 0   /* N = 1 lane; N = 2 group while its waves fit one per SIMD, else lane; N >= 3 group */
 #define CBC_MAP_LANE 1   /* one system per lane, whole state in VGPRs (N = 1..4) */
-#define CBC_MAP_GROUP 2  /* TILE lanes per system (2 x TILE for N = 9..16), coupling solve staged in LDS (N = 2..16) */
+#define CBC_MAP_GROUP 2  /* TILE lanes per system (16 for N = 5..8, a wave for N = 9..16), coupling solve across the group (N = 2..16) */
 
 /* coupling solve */
 #define CBC_LIN_DIRECT 0      /* LU with partial pivoting */
