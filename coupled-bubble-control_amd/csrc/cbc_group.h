@@ -411,7 +411,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
             const double key = used ? -1.0 : pivot_key(ar[k], me);
             // every candidate lane holds the same maximum; lane 0 is one of them
             const int row = __builtin_amdgcn_readfirstlane(pivot_key_row(group_max_wide<32>(key)));
-            const int src = row < N ? row : (row < 2 * N ? 16 + row - N : 0);  // owner lane (63: none)
+            const int src = row < N ? row : (row < 2 * N ? GroupShape<N>::T + row - N : 0);  // owner lane (63: none)
             const bool own = cand && row == me;
             const double inv = readlane_double(rcp_fast(ar[k]), src);
             inv_me = own ? inv : inv_me;
