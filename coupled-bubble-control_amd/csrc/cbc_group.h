@@ -1,23 +1,22 @@
 // cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
 // one lane per bubble, 64/TILE systems per wave (one wave per workgroup), for
-// N <= 4; for N = 5..16 two lanes per bubble (16 lanes per system for N = 5..8,
-// one system per wave for N = 9..16).
+// N = 2, 3; for N = 4..16 two lanes per bubble (8 lanes per system for N = 4, 16
+// for N = 5..8, one system per wave for N = 9..16).
 //
 // This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
 // with a different coupling solve: instead of the block-synchronous Richardson
-// sweep (ckm1d_def.cuh:252-376), for N = 5..16 lane i of a group owns row i
+// sweep (ckm1d_def.cuh:252-376), for N = 4..16 lane i of a group owns row i
 // (radial) and lane TILE + i row N + i (translational) of the 2N x (2N+1)
 // augmented system in VGPRs, both carrying bubble i, and the group runs
 // Gauss--Jordan with partial pivoting: per column the pivot is the row with
 // the largest |a_ik| among rows not yet used (ties: lowest row index), found by one
 // max across the group over keys that pack |a|, the row and the sign (entries
-// closer than 2^-45 relative count as ties); its owner stages it in LDS (double
-// buffered, one wavefront-scope sync per column) and every lane eliminates that
-// column from its row (for N = 9..16, one system per wave, the owner's row
-// reaches the lanes by v_readlane instead).  For N <= 4 (n <= 8) lane i builds both rows of bubble i,
-// the group stages its n rows in LDS once and every lane solves the whole system
-// in registers with the lane mapping's LU (identical results on every lane, one
-// sync instead of n).
+// closer than 2^-45 relative count as ties); its owner's row reaches the other
+// lanes by ds_bpermute (for N = 9..16, one system per wave, by v_readlane) and
+// every lane eliminates that column from its row.  For N = 2, 3 lane i builds
+// both rows of bubble i, the group stages its n rows in LDS once (pairs: by DPP)
+// and every lane solves the whole system in registers with the lane mapping's LU
+// (identical results on every lane, one sync instead of n).
 // Bubble data needed by the coupling sums (x, H0..H6) and the
 // coupling constants mx are staged in LDS per system; pairs (N = 2) pass the
 // bubble data and the rows by DPP broadcasts instead.  All lanes of a group
@@ -39,7 +38,7 @@ constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2);
 template <int N>
 struct GroupShape {
     static constexpr int T = group_tile(N);   // lanes per bubble set: the buffers' TILE
-    // N = 5..16: two lanes per bubble, lane i (i < T) with its radial coupling row
+    // N = 4..16: two lanes per bubble, lane i (i < T) with its radial coupling row
     // and lane T + i with its translational row; both hold the bubble's state and
     // step it identically, and only the first stores it.  N = 9..16 (`wave`): one
     // system per wave, lanes 32 + i and 48 + i carry bubble i once more (same
@@ -49,7 +48,7 @@ struct GroupShape {
     static constexpr int G = group_lanes(N);  // lanes per system
     static constexpr int S = 64 / G;          // systems per wave
     static constexpr int n = 2 * N;           // coupling unknowns
-    static constexpr bool redundant = n <= 8;  // every lane solves the whole system
+    static constexpr bool redundant = !split;  // every lane solves the whole system (N <= 3)
 };
 
 template <int N>
@@ -59,9 +58,8 @@ struct GroupLds {
     static constexpr bool R = GroupShape<N>::redundant;
     // Most frequently accessed first: ds_read2/ds_write2 address two doubles at
     // 8-bit offsets in units of 8 bytes (up to 2040 bytes from the base
-    // register), so the per-column pivot rows and the bubble data stay in reach
-    // of the group's base address without an address add per access.
-    double piv[2][R ? 1 : n + 1];   // pivot row, double buffered (Gauss--Jordan)
+    // register), so the staged rows and the bubble data stay in reach of the
+    // group's base address without an address add per access.
     double sol[R ? 1 : n];          // solved accelerations (Gauss--Jordan)
     double rows[R ? n : 1][n + 1];  // the augmented system (redundant solve)
     double xh[8][G];           // x, H0..H6 of every bubble
@@ -191,15 +189,13 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One entry of a Gauss--Jordan pivot row into LDS.  Merged into ds_write2_b64
-// pairs, these stores made the backend copy both data operands into a fresh
-// register pair first (four extra moves per pair); a volatile LDS store stays a
-// single ds_write_b64 from the row's own registers (N = 8: an attempt 45.0 ->
-// 43.6 us, end states bit for bit the same).
-__device__ __forceinline__ void pivot_store(double* p, double v)
+// Value of v in the lane at byte address `addr` (4 x lane) for every lane: two
+// ds_bpermute (the LDS crossbar; every lane of the wave active).
+__device__ __forceinline__ double bperm_double(int addr, double v)
 {
-    typedef __attribute__((address_space(3))) volatile double lds_vdouble;
-    *(lds_vdouble*)p = v;
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
 }
 
 // Value of v in lane `lane` (wave-uniform) for every lane.
@@ -428,40 +424,38 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         dy[2] = L.sol[ii];
         dy[3] = L.sol[N + ii];
         } else {
-        // Gauss--Jordan with partial pivoting, one row per lane (N = 5..8, 16
+        // Gauss--Jordan with partial pivoting, one row per lane (N = 4..8, 8 or 16
         // lanes): per column one max over the pivot keys of the group (DPP steps
-        // within the 16-lane row, then for 32 lanes the group's two rows exchanged),
-        // the owner stages its row with its exact 1 / a_kk in slot k, every lane
-        // eliminates the column from its row.  Every solution slot starts as NaN
-        // (the sync before this solve ordered it after the previous solve's
-        // reads): a column that finds no finite unused row (NaN entries) stages no
-        // row, so some slot is never written and the accelerations come out NaN,
-        // never stale values of the previous solve.
+        // within the 16-lane row); the owner's exact 1 / a_kk and row reach the
+        // group's lanes by ds_bpermute from the owner's lane (the LDS crossbar, no
+        // LDS stores, sync or loads), and every lane eliminates the column from its
+        // row.  Every solution slot starts as NaN (the sync before this solve
+        // ordered it after the previous solve's reads): a column that finds no
+        // finite unused row (NaN entries) has no owner, so some slot is never
+        // written and the accelerations come out NaN, never stale values of the
+        // previous solve.
         const int me = half ? N + i : i;
         if (vu) L.sol[me] = __builtin_nan("");
         bool used = !vu;
         int col = 0;
         double inv_me = 0.0;
+        const int gbase = ((int)threadIdx.x / GroupShape<N>::G) * GroupShape<N>::G;
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-            double* pv = L.piv[k & 1];
             const double key = used ? -1.0 : pivot_key(ar[k], me);
             const int row = pivot_key_row(group_max_wide<GroupShape<N>::G>(key));
             const bool own = vu && row == me;
-            const double inv_own = rcp_fast(ar[k]);
+            // byte address of the owner's lane for ds_bpermute (row 63: none)
+            const int addr = 4 * (gbase + (row < N ? row : (row < 2 * N ? GroupShape<N>::T + row - N : 0)));
+            const double inv = bperm_double(addr, rcp_fast(ar[k]));
+            inv_me = own ? inv : inv_me;
             if (own) {
-                pivot_store(pv + k, inv_own);
-#pragma unroll
-                for (int j = k + 1; j <= n; ++j) pivot_store(pv + j, ar[j]);
                 used = true;
                 col = k;
             }
-            wave_sync();
-            const double inv = pv[k];
-            inv_me = own ? inv : inv_me;
             const double f = own ? 0.0 : ar[k] * inv;
 #pragma unroll
-            for (int j = k + 1; j <= n; ++j) ar[j] -= f * pv[j];
+            for (int j = k + 1; j <= n; ++j) ar[j] -= f * bperm_double(addr, ar[j]);
         }
         if (vu) L.sol[col] = ar[n] * inv_me;
         wave_sync();
