@@ -13,7 +13,7 @@ in the reference buffer layout) and integrates all of them together:
   (cbc_launch_fused): each wave drains the bins' queues in turn, largest bubble
   count first, so the longest attempt chains start first and waves that find a
   bin drained move on to the next.  The step of the 32 768-system mixed set is
-  its slowest N = 8 system's chain either way (5.43 s fused, 5.48 s with the
+  its slowest N = 8 system's chain either way (5.04 s fused, 5.00 s with the
   per-bin launches below, under the profiler); the bins' results are bit for
   bit those of their own launches;
 * otherwise (or with fused="off") each bin runs one persistent launch
