@@ -270,3 +270,42 @@ def test_fused_selection_and_errors(device):
     after = _snapshot(ml)
     for k in before:
         np.testing.assert_array_equal(before[k], after[k], err_msg=k)
+
+
+@pytest.mark.parametrize("case", ["nk1-sw_a", "const-exact"])
+def test_fused_other_specs(device, case):
+    """The fused kernels for one harmonic and the other field types, and the exact
+    collision-event mode through a fused launch: bit for bit the per-bin launches."""
+    import dataclasses
+
+    from coupledbubble_control import workloads
+    from coupledbubble_control.ensemble import BinTuning, MixedEnsemble
+
+    print(f"case: fused {case}", flush=True)
+    spec = ((8, 16), (2, 96)) if case == "nk1-sw_a" else ((4, 40), (3, 48), (2, 96))
+    ws = _fused_bins(spec, seed=95)
+    for w in ws:
+        if case == "nk1-sw_a":
+            w.PA, w.FR, w.PS = w.PA[:1], w.FR[:1], w.PS[:1]
+            w.field = "SW_A"
+        else:
+            w.field = "CONST"
+
+    def build():
+        ms = [workloads.build_model(w) for w in ws]
+        if case == "const-exact":
+            for m in ms:
+                m._solver_spec = dataclasses.replace(m._solver_spec, event_mode="exact", etol=1e-12)
+        return ms
+
+    tune = {n: BinTuning("group") for n, _ in spec}
+    a, b = build(), build()
+    MixedEnsemble(a, tune, fused="off").solve()
+    eb = MixedEnsemble(b, tune, fused="on")
+    eb.solve()
+    assert eb.fused_active
+    for ma, mb in zip(a, b):
+        sa, sb = _snapshot(ma), _snapshot(mb)
+        assert (sa["flags"] != 0).all()
+        for k in sa:
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=f"N={ma._execution_spec.ups}: {k}")
