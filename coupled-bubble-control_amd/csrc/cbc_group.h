@@ -1,11 +1,11 @@
 // cbc_group.h -- group mapping: TILE = next_pow2(N) lanes per CKM1D system,
 // one lane per bubble, 64/TILE systems per wave (one wave per workgroup), for
-// N = 2, 3; for N = 4..16 two lanes per bubble (8 lanes per system for N = 4, 16
-// for N = 5..8, one system per wave for N = 9..16).
+// N = 2..4; for N = 5..16 two lanes per bubble (16 lanes per system for N = 5..8,
+// one system per wave for N = 9..16).
 //
 // This is the reference's thread-per-bubble layout (rkck45_coupled_cuda_sover.cu:228-239)
 // with a different coupling solve: instead of the block-synchronous Richardson
-// sweep (ckm1d_def.cuh:252-376), for N = 4..16 lane i of a group owns row i
+// sweep (ckm1d_def.cuh:252-376), for N = 5..16 lane i of a group owns row i
 // (radial) and lane TILE + i row N + i (translational) of the 2N x (2N+1)
 // augmented system in VGPRs, both carrying bubble i, and the group runs
 // Gauss--Jordan with partial pivoting: per column the pivot is the row with
@@ -13,7 +13,7 @@
 // max across the group over keys that pack |a|, the row and the sign (entries
 // closer than 2^-45 relative count as ties); its owner's row reaches the other
 // lanes by ds_bpermute (for N = 9..16, one system per wave, by v_readlane) and
-// every lane eliminates that column from its row.  For N = 2, 3 lane i builds
+// every lane eliminates that column from its row.  For N = 2..4 lane i builds
 // both rows of bubble i, the group stages its n rows in LDS once (pairs: by DPP)
 // and every lane solves the whole system in registers with the lane mapping's LU
 // (identical results on every lane, one sync instead of n).
@@ -38,7 +38,7 @@ constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2);
 template <int N>
 struct GroupShape {
     static constexpr int T = group_tile(N);   // lanes per bubble set: the buffers' TILE
-    // N = 4..16: two lanes per bubble, lane i (i < T) with its radial coupling row
+    // N = 5..16: two lanes per bubble, lane i (i < T) with its radial coupling row
     // and lane T + i with its translational row; both hold the bubble's state and
     // step it identically, and only the first stores it.  N = 9..16 (`wave`): one
     // system per wave, lanes 32 + i and 48 + i carry bubble i once more (same
@@ -48,7 +48,7 @@ struct GroupShape {
     static constexpr int G = group_lanes(N);  // lanes per system
     static constexpr int S = 64 / G;          // systems per wave
     static constexpr int n = 2 * N;           // coupling unknowns
-    static constexpr bool redundant = !split;  // every lane solves the whole system (N <= 3)
+    static constexpr bool redundant = !split;  // every lane solves the whole system (N <= 4)
 };
 
 template <int N>
@@ -424,7 +424,7 @@ __device__ __forceinline__ void group_rhs(const GroupParams<NK>& P, GroupLds<N>&
         dy[2] = L.sol[ii];
         dy[3] = L.sol[N + ii];
         } else {
-        // Gauss--Jordan with partial pivoting, one row per lane (N = 4..8, 8 or 16
+        // Gauss--Jordan with partial pivoting, one row per lane (N = 5..8, 16
         // lanes): per column one max over the pivot keys of the group (DPP steps
         // within the 16-lane row); the owner's exact 1 / a_kk and row reach the
         // group's lanes by ds_bpermute from the owner's lane (the LDS crossbar, no

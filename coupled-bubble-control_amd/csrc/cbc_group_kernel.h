@@ -9,7 +9,7 @@
 // compute but never store.
 //
 // Plain mode: group = system blockIdx.x * S + group index.  Lane i of a group
-// carries bubble i; in the groups of N = 4..16, lanes i and TILE + i both carry
+// carries bubble i; in the groups of N = 5..16, lanes i and TILE + i both carry
 // bubble i (its radial and its translational coupling row; for N = 9..16 lanes
 // 32 + i and 48 + i a copy of them) and step it identically, and lane i alone
 // stores its results.

@@ -99,15 +99,15 @@ hipError_t lane_attr(int ups, int nk, int field, KernelAttr* out);
 bool lane_supported(int ups, int nk, int field);
 
 // Group mapping, one wave per workgroup: the buffers' TILE = next_pow2(ups) lanes
-// per system, one per bubble (ups = 2, 3, whose lanes each solve the whole coupling
-// system); for ups = 4..8 two lanes per bubble (8 or 16 per system), one per
-// coupling row; for ups = 9..16 one system per wave (64 lanes: the two row lanes of
-// each bubble, and a copy of them that carries the bubble without taking part in
-// the solve).
+// per system, one per bubble (ups = 2..4, whose lanes each solve the whole coupling
+// system); for ups = 5..8 two lanes per bubble (16 per system), one per coupling
+// row; for ups = 9..16 one system per wave (64 lanes: the two row lanes of each
+// bubble, and a copy of them that carries the bubble without taking part in the
+// solve).
 constexpr int group_tile(int ups) { return ups <= 1 ? 1 : 2 * group_tile((ups + 1) / 2); }
 constexpr int group_lanes(int ups)
 {
-    return group_tile(ups) == 16 ? 64 : (group_tile(ups) == 8 || ups == 4) ? 2 * group_tile(ups) : group_tile(ups);
+    return group_tile(ups) == 16 ? 64 : group_tile(ups) == 8 ? 16 : group_tile(ups);
 }
 
 hipError_t group_launch(const LaunchArgs& a, hipStream_t stream);
