@@ -110,10 +110,19 @@ __device__ __forceinline__ void group_steps(Step step)
     if constexpr (G >= 16) step(std::integral_constant<int, kDppMirror>{});
 }
 
+// Minimum of two doubles, neither a signalling NaN (as max_quiet below).
+__device__ __forceinline__ double min_quiet(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <int G>
 __device__ __forceinline__ double group_min(double v)
 {
-    group_steps<G>([&](auto c) { v = fmin(v, dpp_double<decltype(c)::value>(v)); });
+    v = __builtin_canonicalize(v);
+    group_steps<G>([&](auto c) { v = min_quiet(v, dpp_double<decltype(c)::value>(v)); });
     return v;
 }
 
@@ -143,10 +152,27 @@ __device__ __forceinline__ int pivot_key_row(double key)
     return 63 - (int)(((unsigned long long)__double_as_longlong(key) & 0x7Full) >> 1);
 }
 
+// Maximum of two doubles, neither a signalling NaN, as one v_max_f64 (IEEE
+// mode: a quiet NaN operand yields the other operand, as fmax).  fmax of an
+// operand the compiler cannot prove quiet -- a DPP or permlane result -- first
+// quiets it with a maximum of the operand with itself, a second FP64 operation
+// on every step of a reduction; the group reductions quiet their value once and
+// every later operand is a maximum (minimum) or a moved one, so the results are
+// those of fmax (fmin) bit for bit.  The compiler's hazard check counts the statement as a VALU
+// write and puts the wait states a DPP, permlane or readlane read of the result
+// needs after it, as after its own v_max_f64.
+__device__ __forceinline__ double max_quiet(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <int G>
 __device__ __forceinline__ double group_max(double v)
 {
-    group_steps<G>([&](auto c) { v = fmax(v, dpp_double<decltype(c)::value>(v)); });
+    v = __builtin_canonicalize(v);
+    group_steps<G>([&](auto c) { v = max_quiet(v, dpp_double<decltype(c)::value>(v)); });
     return v;
 }
 
@@ -161,7 +187,7 @@ __device__ __forceinline__ double group_max_wide(double v)
         v = group_max<16>(v);
         const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
         const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
-        return fmax(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
+        return max_quiet(__hiloint2double(hi[0], lo[0]), __hiloint2double(hi[1], lo[1]));
     } else {
         return group_max<G>(v);
     }
