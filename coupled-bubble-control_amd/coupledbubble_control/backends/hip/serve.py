@@ -28,6 +28,7 @@ class ServeOptions:
     waves: int = 0               # persistent grid in waves (0: lanes for min(capacity, ns) systems)
     refill: int = 1              # free lanes (lane groups) of a wave that trigger a claim
     poll_interval: int = 8       # attempts between ring polls of a wave with live systems
+                                 # (up to 8x that after polls that claim nothing; 1 .. 2^20)
     max_attempts: int = 1 << 20  # per item; a system reaching it finishes ACTIVE
     idle_us: int = 200000        # idle waves exit after this long without work
     copy_state: bool = True      # copy end states to the host-mapped slab
@@ -44,6 +45,7 @@ class ServeQueue:
     def __init__(self, model, opts: Optional[ServeOptions] = None, stream: Optional[torch.cuda.Stream] = None):
         """``stream=None`` lets the library create the serving stream with a hardware queue of
         its own (recommended: the serving kernel runs until it idles out)."""
+        self._h = None
         self.model = model
         self.opts = opts or ServeOptions()
         model._ensure_buffers()

@@ -672,7 +672,8 @@ int32_t cbc_serve_create(cbc_plan* plan, const cbc_buffers* buf, const cbc_serve
             return fail(CBC_ERR_INVALID, "capacity must be a power of two in [64, 2^24], got %d", cap);
         if (spec->refill < 1 || spec->refill > 64)
             return fail(CBC_ERR_INVALID, "refill must be in [1, 64], got %d", spec->refill);
-        if (spec->poll_interval < 1) return fail(CBC_ERR_INVALID, "poll_interval must be >= 1");
+        if (spec->poll_interval < 1 || spec->poll_interval > (1 << 20))
+            return fail(CBC_ERR_INVALID, "poll_interval must be in [1, 2^20], got %d", spec->poll_interval);
         if (spec->max_attempts < 1) return fail(CBC_ERR_INVALID, "max_attempts must be >= 1");
         if (spec->idle_us < 1 || spec->idle_us > 10000000)
             return fail(CBC_ERR_INVALID, "idle_us must be in [1, 1e7], got %d", spec->idle_us);

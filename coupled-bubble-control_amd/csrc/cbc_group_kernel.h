@@ -431,6 +431,7 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
     const bool persistent = a.queue != nullptr || serving;
     bool drained = !persistent;
     int since_poll = 0x40000000;
+    int poll_shift = 0;
     uint64_t idle_since = 0;
     int idle_rounds = 0;
     if (!persistent) {
@@ -447,7 +448,7 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
             const int nfree = __popcll(freemask);
             const bool any_busy = __any(busy);
             bool take = nfree > 0 && (nfree >= a.refill || !any_busy);
-            if (serving) take = take && (!any_busy || since_poll >= a.poll_interval);
+            if (serving) take = take && (!any_busy || since_poll >= (a.poll_interval << poll_shift));
             if (take) {
                 const int leader = __ffsll((unsigned long long)freemask) - 1;
                 const uint32_t rank = (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
@@ -458,6 +459,7 @@ __device__ __forceinline__ void group_run(const LaunchArgs& a, int block, GroupL
                     if (lane == leader) got = serve_claim(a, (uint32_t)nfree, &base);
                     got = __shfl(got, leader);
                     base = __shfl(base, leader);
+                    poll_shift = poll_shift_after(poll_shift, got);
                     if (got > 0) fence_acquire_system();
                     int slot = (free_leader && rank < got) ? (int)rank : -1;
                     slot = __shfl(slot, ls * G);  // group leader's slot to the whole group

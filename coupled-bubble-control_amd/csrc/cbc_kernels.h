@@ -63,6 +63,7 @@ struct LaunchArgs {
     const uint32_t* ring_stop;   // host-mapped: nonzero = exit once idle
     uint32_t ring_mask;          // ring capacity - 1 (capacity a power of two)
     int32_t poll_interval;       // attempts between ring polls of a wave with free lanes
+                                 // (up to 8x that after polls that claim nothing)
     uint64_t idle_ticks;         // wall-clock ticks an idle wave waits for work before exiting
     uint32_t* done_seq;          // host-mapped (NS): seq of the last item finished per system
     double* out_state;           // host-mapped (4, NS, TILE) final state copy (may be null)

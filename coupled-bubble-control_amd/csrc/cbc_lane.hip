@@ -599,7 +599,8 @@ __device__ __forceinline__ void lane_finish(const LaunchArgs& a, const LaneRun<N
 // at least `refill` lanes are free, or all are; the wave exits when the queue is
 // drained and its lanes are idle.  Serving mode: as persistent mode, but items
 // come from the host-mapped ring (polled every `poll_interval` attempts while
-// some lane is busy), and an idle wave waits for work until the host stops it
+// some lane is busy, up to 8x as often after polls that found nothing:
+// poll_shift_after), and an idle wave waits for work until the host stops it
 // or `idle_ticks` pass.  All modes run the same attempt loop.
 template <int N, int NK, int FIELD>
 __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
@@ -611,6 +612,7 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
     const bool persistent = a.queue != nullptr || serving;
     bool drained = !persistent;
     int since_poll = 0x40000000;
+    int poll_shift = 0;
     uint64_t idle_since = 0;
     int idle_rounds = 0;
     if (!persistent) {
@@ -623,7 +625,7 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
             const uint64_t freemask = __ballot(!busy);
             const int nfree = __popcll(freemask);
             bool take = nfree > 0 && (nfree >= a.refill || nfree == 64);
-            if (serving) take = take && (nfree == 64 || since_poll >= a.poll_interval);
+            if (serving) take = take && (nfree == 64 || since_poll >= (a.poll_interval << poll_shift));
             if (take) {
                 const int leader = __ffsll((unsigned long long)freemask) - 1;
                 const uint32_t rank = (uint32_t)__popcll(freemask & ((1ull << lane) - 1ull));
@@ -633,6 +635,7 @@ __global__ void __launch_bounds__(256) lane_kernel(const LaunchArgs a)
                     if (lane == leader) got = serve_claim(a, (uint32_t)nfree, &base);
                     got = __shfl(got, leader);
                     base = __shfl(base, leader);
+                    poll_shift = poll_shift_after(poll_shift, got);
                     if (got > 0) fence_acquire_system();
                     if (!busy && rank < got) {
                         const ServeItem it = serve_item(a, base + rank);

@@ -104,6 +104,20 @@ __device__ __forceinline__ void serve_publish_flag(const LaunchArgs& a, int s, u
     sys_store_u32(a.done_seq + s, seq);
 }
 
+// A wave that still has live systems polls the ring every
+// poll_interval << shift attempts: a poll that claims no item doubles the
+// interval, up to kPollBackoff doublings, and one that claims an item resets it.
+// Under an offered load most such polls find the ring drained (items arrive in
+// bursts, far apart on the attempt scale) and each is a read of host-mapped
+// memory; a burst is still taken up within a few polls, since hundreds of waves
+// poll at random phases and each claims as many items as it has free lanes.
+constexpr int kPollBackoff = 3;
+
+__device__ __forceinline__ int poll_shift_after(int shift, uint32_t got)
+{
+    return got > 0u ? 0 : (shift < kPollBackoff ? shift + 1 : kPollBackoff);
+}
+
 // Idle handling of a wave with no live system.  Returns true when the wave
 // should exit.  `idle_since` is 0 while the wave is busy.  Every idle round
 // reads the ring (tail and claim counter) before it gets here, and those reads

@@ -257,7 +257,9 @@ typedef struct cbc_serve_spec {
     int32_t capacity;       /* ring slots: items queued or running at once (power of two, >= 64) */
     int32_t waves;          /* persistent grid in waves (0 = lanes for min(capacity, ns) systems, at most one wave per SIMD slot) */
     int32_t refill;         /* free lanes (groups) of a wave that trigger a claim, 1..64 */
-    int32_t poll_interval;  /* attempts between ring polls of a wave that has live systems */
+    int32_t poll_interval;  /* attempts between ring polls of a wave that has live systems
+                               (doubling after each poll that claims nothing, up to 8x);
+                               1 .. 2^20 */
     int64_t max_attempts;   /* per item; a system reaching it finishes with CBC_ACTIVE */
     int32_t idle_us;        /* an idle wave exits after this many microseconds without work */
     int32_t copy_state;     /* nonzero: finished items also copy their state to out_state */

@@ -255,6 +255,9 @@ def test_serve_relaunch_after_idle_and_admission():
 
     print("\n[relaunch/admission] N=2 ns=256", flush=True)
     _, m, _ = _models(2, 256, seed=14)
+    for bad in (0, (1 << 20) + 1):   # the kernels shift the interval left by up to 3
+        with pytest.raises(_abi.CBCError):
+            ServeQueue(m, ServeOptions(capacity=128, poll_interval=bad))
     with ServeQueue(m, ServeOptions(capacity=128, max_attempts=200000, idle_us=500)) as q:
         ids = np.arange(64)
         pa = np.full((2, 64), 0.5e5)
