@@ -98,10 +98,10 @@ PARAM_STUDY_TRAFFIC_BYTES_PER_SYSTEM_200 = 3955
 # rocprofv3 over a default serving run (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of
 # a copy kernel of known size), total bytes of the serving dispatches of the per-env-ordered
 # continuous run / its env steps; ring, control words and outputs live in mapped host
-# memory, so this bounds the HBM bytes from above: 1.47 GB read + 0.27 GB written over
-# 458 752 env steps (`--steps 20 --warmup 5 --serving-plain 0`), i.e. 3 214 + 584 bytes
+# memory, so this bounds the HBM bytes from above: 1.28 GB read + 0.27 GB written over
+# 458 752 env steps (`--steps 20 --warmup 5 --serving-plain 0`), i.e. 2 784 + 584 bytes
 # (profiles/serving_16384_N2x12288_N4x4096.md).
-SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 3798
+SERVING_TRAFFIC_BYTES_PER_ENV_STEP = 3368
 METRIC = "acoustic periods/s (env-steps/s) on 1 GPU; p50/p99 step latency vs plain"
 
 

@@ -72,20 +72,27 @@ __device__ __forceinline__ uint32_t serve_claim(const LaunchArgs& a, uint32_t wa
     return 0u;
 }
 
-// Item `idx` of the ring (system-scope loads: the slot is rewritten by the host
-// between uses, so no cached copy may be used).
+// Item `idx` of the ring as three 16-byte loads (the 48-byte items start at
+// 16-byte multiples of the page-aligned ring).  The slot is rewritten by the host
+// between uses; the loads come after the claim's system-scope acquire fence,
+// which orders them after the tail read that covered the item (the host writes
+// the item, then publishes the tail with a release store) and invalidates
+// cached lines, so they read what the host published.  Seven system-scope loads,
+// one per field, each crossed the host link on their own.
 __device__ __forceinline__ ServeItem serve_item(const LaunchArgs& a, uint32_t idx)
 {
-    const ServeItem* p = a.ring + (idx & a.ring_mask);
+    static_assert(sizeof(ServeItem) == 48 && alignof(ServeItem) <= 16, "ServeItem layout");
+    const uint4* p = reinterpret_cast<const uint4*>(a.ring + (idx & a.ring_mask));
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2];
     ServeItem it;
-    it.sys = sys_load_i32(&p->sys);
-    it.seq = sys_load_u32(&p->seq);
-    it.opts = sys_load_i32(&p->opts);
+    it.sys = (int32_t)w0.x;
+    it.seq = w0.y;
+    it.opts = (int32_t)w0.z;
     it.pad = 0;
-    it.t_end = sys_load_f64(&p->t_end);
-    it.h0 = sys_load_f64(&p->h0);
-    it.pa[0] = sys_load_f64(&p->pa[0]);
-    it.pa[1] = sys_load_f64(&p->pa[1]);
+    it.t_end = __hiloint2double((int)w1.y, (int)w1.x);
+    it.h0 = __hiloint2double((int)w1.w, (int)w1.z);
+    it.pa[0] = __hiloint2double((int)w2.y, (int)w2.x);
+    it.pa[1] = __hiloint2double((int)w2.w, (int)w2.z);
     return it;
 }
 
