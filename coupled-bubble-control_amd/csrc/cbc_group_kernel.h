@@ -277,8 +277,10 @@ __device__ __forceinline__ bool group_attempt(const LaunchArgs& a, int i, int ha
     const int s = R.s;
     const bool crossing = R.g_old * g_new < 0.0 && !(a.event_exact && fabs(g_new) < a.etol);
     if (crossing) {
-        if (h <= a.min_step) {
-            R.flag = kStepSizeLimit;
+        // no progress possible; a NaN step size (the retry factor of an infinite gap)
+        // passes every clamp and would be retried until the attempt budget runs out
+        if (!(h > a.min_step)) {
+            R.flag = h == h ? kStepSizeLimit : kNaN;
             return true;
         }
         if (a.event_exact) {

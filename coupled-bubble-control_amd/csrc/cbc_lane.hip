@@ -449,8 +449,10 @@ __device__ __forceinline__ bool lane_attempt(const LaunchArgs& a, LaneRun<N, NK>
     // exact mode a crossing step that lands within etol of the root is accepted
     const bool crossing = R.g_old * g_new < 0.0 && !(a.event_exact && fabs(g_new) < a.etol);
     if (crossing) {
-        if (h <= a.min_step) {  // cannot shrink further: no progress possible
-            R.flag = kStepSizeLimit;
+        // cannot shrink further: no progress possible; a NaN step size (the retry
+        // factor of an infinite gap) passes every clamp and would never end
+        if (!(h > a.min_step)) {
+            R.flag = h == h ? kStepSizeLimit : kNaN;
             return true;
         }
         double ratio;

@@ -521,8 +521,11 @@ int or_advance(const or_system *S, const or_tol *T, or_run *st, double *y, int k
         for (int c = 0; c < d; c++) if (!isfinite(ynew[c])) nan_state = 1;
 
         if (crossing) {
-            if (h <= T->min_step) {       /* no progress possible */
-                st->flag = OR_STEP_SIZE_LIMIT;
+            /* no progress possible; a NaN step size (the retry factor of an infinite
+               gap) passes every clamp, and the reference retries it until its step
+               budget runs out: flagged as a non-finite step instead */
+            if (!(h > T->min_step)) {
+                st->flag = (h == h) ? OR_STEP_SIZE_LIMIT : OR_NAN;
                 st->t = t; st->h = h; st->event = g_old;
                 or_window_flush(P, wcur, t, n, wmax, wsum);
                 return attempts;

@@ -46,6 +46,38 @@ def test_group_mapping_parity(device, n, ns, periods):
     _assert_counts(m, cnt)
 
 
+@pytest.mark.parametrize("n", [8, 16])
+def test_non_finite_system_is_flagged_alone(device, n):
+    """A NaN radius in one system and an infinite one in another (coupling entries that
+    are infinite give the pivot search keys that are signalling NaNs until quieted; the
+    starting gap -inf makes the first trial step a crossing whose retry factor is NaN)
+    stop those systems with SINGULARITY_OR_NAN within a few attempts; every other system
+    of their waves ends bit for bit as in a run without them (the Gauss--Jordan kernels
+    reduce within each group).  max_steps bounds a regression to seconds."""
+    from coupledbubble_control import workloads
+    from coupledbubble_control.backends import KernelVariant
+
+    print(f"case: non-finite injection N={n} 8 systems", flush=True)
+    w = _any_n(n, 8, 0.25, seed=21)
+    runs = []
+    for inject in (False, True):
+        m = workloads.build_model(w, variant=KernelVariant.GROUP)
+        if inject:
+            m.runtime.host_buffers.state.actual_state[0, 1, 2] = np.nan
+            m.runtime.host_buffers.state.actual_state[0, 6, 0] = np.inf
+        m.solve(max_steps=200000, kernel_steps=2048, sync_to_host=True)
+        hb = m.runtime.host_buffers
+        assert m.runtime.kernel.info().mapping == "group"
+        runs.append((hb.status.status_flags.copy(), hb.state.actual_state.copy(), hb.state.actual_time.copy()))
+    (f0, y0, t0), (f1, y1, t1) = runs
+    assert f1[1] == -4 and f1[6] == -4
+    rest = np.array([s for s in range(8) if s not in (1, 6)])
+    assert np.all(f0 != -4)
+    np.testing.assert_array_equal(f1[rest], f0[rest])
+    np.testing.assert_array_equal(y1[:, rest], y0[:, rest])
+    np.testing.assert_array_equal(t1[rest], t0[rest])
+
+
 @pytest.mark.parametrize("n,ns,periods", [(5, 48, 1.0), (6, 40, 1.0), (7, 32, 1.0), (9, 24, 0.5), (10, 16, 0.5),
                                           (11, 16, 0.5), (12, 16, 0.5), (13, 12, 0.5), (14, 12, 0.5), (15, 12, 0.5)])
 def test_group_mapping_other_bubble_counts(device, n, ns, periods):

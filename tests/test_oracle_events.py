@@ -64,3 +64,26 @@ def test_bracket_mode_stops_before_the_root():
         flag, t, g = _oracle_event(w, s, 0.1, "bracket", 1e-5, 1e-9)
         assert flag == 2 and 0.0 < g < 1e-5
         assert t_ref * (1 - 1e-4) < t < t_ref
+
+
+def test_infinite_radius_stops_as_non_finite():
+    """An infinite radius makes the starting gap -inf, so the first trial step counts as a
+    crossing and the bracket retry's factor inf/inf gives a NaN step size, which passes
+    every clamp; it is flagged SINGULARITY_OR_NAN (-4) at the next crossing instead of
+    being retried until the step budget runs out (65 536 attempts, ACTIVE, before the
+    check).  A NaN radius stops the same way; the other systems are unaffected."""
+    w = workloads.plain_cpu(seed=0).subset(np.arange(4))
+    clean = O.host_buffers_dict(workloads.build_model(w))
+    m = workloads.build_model(w)
+    b = O.host_buffers_dict(m)
+    b["actual_state"][0, 1, 0] = np.inf
+    b["actual_state"][0, 2, 1] = np.nan
+    sp = O.spec_from_model(m)
+    r = O.solve(sp, b, kernel_steps=2048, max_steps=1 << 16)
+    r0 = O.solve(O.spec_from_model(m), clean, kernel_steps=2048, max_steps=1 << 16)
+    assert b["status_flags"][1] == -4 and b["status_flags"][2] == -4
+    assert r["total_steps"][1] <= 4 and r["total_steps"][2] <= 4
+    for s in (0, 3):
+        assert b["status_flags"][s] == clean["status_flags"][s] >= 1
+        assert r["total_steps"][s] == r0["total_steps"][s]
+        np.testing.assert_array_equal(b["actual_state"][:, s], clean["actual_state"][:, s])
